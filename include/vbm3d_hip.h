@@ -1,0 +1,151 @@
+/*
+ * vbm3d_hip.h -- C ABI of libvbm3d_hip.so, the MI355X (gfx950) VBM3D
+ * collaborative filter.
+ *
+ * Plain C types only: pointers, sizes, a POD parameter block.  No
+ * exception crosses this boundary; every entry point returns VBM3D_OK (0)
+ * or a negative error code, with a message in vbm3d_last_error() (thread
+ * local).  Videos are float32, planar [F][C][H][W] (the reference's
+ * VideoSize layout, src/Utilities/LibVideoT.hpp:88-92); flows are
+ * [F-1 or more][2][H][W] or NULL for zero flow (src/main.cpp:353-369).
+ *
+ * Citations are relative to the reference repository (tehret/vbm3d).
+ */
+#ifndef VBM3D_HIP_H
+#define VBM3D_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VBM3D_OK 0
+#define VBM3D_ERR_ARG (-1)         /* invalid argument or size */
+#define VBM3D_ERR_HIP (-2)         /* HIP runtime error */
+#define VBM3D_ERR_UNSUPPORTED (-3) /* parameters outside what the kernels implement */
+#define VBM3D_ERR_NOMEM (-4)       /* device allocation failed */
+
+/* Transform codes, as accepted by the reference CLI (src/main.cpp:27-30). */
+#define VBM3D_DCT 0
+#define VBM3D_BIOR 1
+#define VBM3D_HADAMARD 2
+#define VBM3D_HAAR 3
+
+/* POD mirror of struct Parameters (src/parameters.h:21-58), field for
+ * field and in the same order. */
+typedef struct vbm3d_params {
+    unsigned k;      /* patch size */
+    unsigned kt;     /* temporal patch size (1 only) */
+    unsigned p;      /* step between reference patches */
+    unsigned N;      /* maximum group size */
+    float tau;       /* distance threshold */
+    float d;         /* bias toward the reference position */
+    unsigned Nf;     /* frames searched before and after */
+    unsigned Ns;     /* search window in the reference frame */
+    unsigned Npr;    /* predictive search window */
+    unsigned Nb;     /* matches kept per frame */
+    unsigned T_2D;   /* VBM3D_DCT or VBM3D_BIOR */
+    unsigned T_3D;   /* VBM3D_HADAMARD or VBM3D_HAAR */
+    float lambda3D;  /* hard threshold (step 1) */
+    int mc;          /* motion-compensated 3D patches (no effect at kt 1) */
+    int n;           /* tile border of the multi-threaded grid (16) */
+} vbm3d_params;
+
+typedef struct vbm3d_ctx vbm3d_ctx;
+
+/* Context owning device memory and a HIP stream on `device`.  Replaces the
+ * per-call FFTW plan setup and per-thread buffers of run_vbm3d
+ * (src/vbm3d.cpp:135-160, 216-217).  NULL on failure. */
+vbm3d_ctx* vbm3d_create(int device);
+void vbm3d_destroy(vbm3d_ctx* ctx);
+const char* vbm3d_last_error(void);
+
+/* Default rules of initializeParameters_1 (src/main.cpp:35-125);
+ * negative arguments mean "not given" (the CLI's NONE = -1).  T_2D/T_3D
+ * take (unsigned)-1 for "not given".  lambda3D, tau: negative = default. */
+void vbm3d_default_params_1(vbm3d_params* p, int k, int kt, int Nf, int Ns,
+                            int Npr, int Nb, int pstep, int N, int d,
+                            float tau, float lambda3D, unsigned T_2D,
+                            unsigned T_3D, float sigma);
+/* Default rules of initializeParameters_2 (src/main.cpp:127-207). */
+void vbm3d_default_params_2(vbm3d_params* p, int k, int kt, int Nf, int Ns,
+                            int Npr, int Nb, int pstep, int N, int d,
+                            float tau, unsigned T_2D, unsigned T_3D,
+                            float sigma);
+
+/* Whole path on host buffers; replaces run_vbm3d (src/vbm3d.cpp:87-376,
+ * declared src/vbm3d.h:25-35).  Semantics kept from the reference:
+ *   - `noisy` is mutated by the forward then inverse colour transform;
+ *   - `basic` is an input when p1->k == 0 (step 1 skipped), else output;
+ *   - p2->k == 0 copies basic to denoised;
+ *   - ref_grid_nb = 1 gives the single-thread reference grid, 2..32
+ *     (power of two) the grid and merge order of the reference run with
+ *     that many OpenMP threads (src/vbm3d.cpp:117-128, 213-354). */
+int vbm3d_run(vbm3d_ctx* ctx, float sigma, float* noisy, const float* fflow,
+              const float* bflow, unsigned W, unsigned H, unsigned F,
+              unsigned C, const vbm3d_params* p1, const vbm3d_params* p2,
+              int color_space, int ref_grid_nb, float* basic,
+              float* denoised);
+
+/* Same on device-resident buffers, enqueued on `stream` (a hipStream_t,
+ * NULL = the context's stream) and synchronised before return.  d_noisy
+ * is mutated like `noisy` above. */
+int vbm3d_run_device(vbm3d_ctx* ctx, float sigma, float* d_noisy,
+                     const float* d_fflow, const float* d_bflow, unsigned W,
+                     unsigned H, unsigned F, unsigned C,
+                     const vbm3d_params* p1, const vbm3d_params* p2,
+                     int color_space, int ref_grid_nb, float* d_basic,
+                     float* d_denoised, void* stream);
+
+/* Predictive block matching only (computeSimilarPatches,
+ * src/vbm3d.cpp:1082-1197) for nrefs reference patches given by index
+ * t*W*H*C + y*W + x, on host buffers.  Per ref: N sorted indices and
+ * distances (entries past nsim are 0xFFFFFFFF / NaN), nsim and the group
+ * size nsx. */
+int vbm3d_block_match(vbm3d_ctx* ctx, const float* vid, unsigned W,
+                      unsigned H, unsigned F, unsigned C, const float* fflow,
+                      const float* bflow, const vbm3d_params* prm,
+                      long nrefs, const uint32_t* ref_idx, uint32_t* idx_out,
+                      float* dist_out, uint32_t* nsim_out, uint32_t* nsx_out);
+
+/* One step on an already colour-transformed video, returning the merged
+ * aggregation buffers (vbm3d_1st_step src/vbm3d.cpp:447-654 when step = 1,
+ * vbm3d_2nd_step :676-928 when step = 2, plus the thread-order merge
+ * :267-280 / :333-346 when ref_grid_nb > 1).  Host buffers. */
+int vbm3d_step(vbm3d_ctx* ctx, int step, float sigma, const float* noisy,
+               const float* basic, const float* fflow, const float* bflow,
+               unsigned W, unsigned H, unsigned F, unsigned C,
+               const vbm3d_params* prm, int color_space, int ref_grid_nb,
+               float* num, float* den);
+
+/* Reference-patch grid of one frame in processing order (crop, row, col);
+ * ind_initialize + subDivideTight (src/Utilities/Utilities.cpp:400-415,
+ * src/Utilities/LibVideoT.hpp:903-961).  Returns the count, or -1. */
+long vbm3d_grid(unsigned W, unsigned H, unsigned k, unsigned p, int n,
+                int nb, unsigned* xs, unsigned* ys, unsigned* crop);
+
+/* Kernel timing and work counters (a diagnostic with no counterpart in
+ * the reference, for roofline measurements).  vbm3d_profile(ctx, 1)
+ * resets the counters and brackets every later launch with HIP events on
+ * the stream it runs on; vbm3d_profile(ctx, 0) stops.  vbm3d_profile_read
+ * returns, for kernel 0 (block matching), 1 (group filtering),
+ * 2 (aggregation) or 3 (colour transform): total milliseconds, number of
+ * launches and the work done -- candidate patches evaluated (0), group
+ * members filtered (1), output samples produced (2).  kernel 4 reads the
+ * block-matching pixel comparisons (candidates * k*k*C; no time). */
+int vbm3d_profile(vbm3d_ctx* ctx, int enable);
+int vbm3d_profile_read(vbm3d_ctx* ctx, int kernel, double* ms, long* launches,
+                       double* work);
+
+/* Host-side check of the group-selection routine the kernels use: sorts
+ * the first `mid` of `len` (dist, idx) pairs like std::partial_sort with
+ * the reference comparator (src/Utilities/Utilities.cpp:40-45). */
+void vbm3d_selftest_partial_sort(float* dist, uint32_t* idx, unsigned len,
+                                 unsigned mid);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,194 @@
+"""GPU parity: libvbm3d_hip.so (through the C ABI) against the CPU oracle.
+
+The bar is bit-exact equality: block-matching indices, distances, group
+sizes, the aggregation sums and the basic / final estimates, on the same
+float32 inputs.  Sizes are chosen so the oracle finishes in seconds.
+"""
+import numpy as np
+import pytest
+
+from vbm3d_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+
+def bits(a):
+    return np.ascontiguousarray(a, np.float32).view(np.uint32)
+
+
+def assert_bits_equal(a, b, what):
+    a = np.asarray(a, np.float32)
+    b = np.asarray(b, np.float32)
+    assert a.shape == b.shape, what
+    neq = bits(a) != bits(b)
+    if neq.any():
+        i = np.argwhere(neq)[0]
+        raise AssertionError(f"{what}: {neq.sum()} of {a.size} differ; first at {tuple(i)}: "
+                             f"{a[tuple(i)]!r} vs {b[tuple(i)]!r}, max|d|={np.abs(a - b).max()}")
+
+
+def all_refs(W, H, F, C, k, p, nb=1):
+    import vbm3d_amd as V
+    xs, ys, _ = V.grid(W, H, k, p, 16, nb)
+    per = (ys.astype(np.int64) * W + xs).astype(np.int64)
+    return np.concatenate([t * W * H * C + per for t in range(F)]).astype(np.uint32)
+
+
+@pytest.fixture(scope="module")
+def cfg1():
+    clean, noisy = synth.noisy_clip(176, 144, 10, 1, 20.0, 0)
+    return clean, noisy
+
+
+@pytest.fixture(scope="module")
+def small_rgb():
+    clean, noisy = synth.noisy_clip(96, 72, 7, 3, 20.0, 11)
+    return clean, noisy
+
+
+def _bm_compare(vid, prm, refs, oracle, ctx, fflow=None, bflow=None):
+    import vbm3d_amd as V
+    gi, gd, gs, gn = V.block_match(vid, prm, refs, fflow, bflow, ctx=ctx)
+    oi, od, os_, on = oracle.block_match(vid, prm, refs, fflow, bflow)
+    np.testing.assert_array_equal(gs, os_)
+    np.testing.assert_array_equal(gn, on)
+    np.testing.assert_array_equal(gi, oi)
+    assert_bits_equal(np.where(np.isnan(gd), 0, gd), np.where(np.isnan(od), 0, od), "distances")
+    return gn
+
+
+def test_bm_config1_step1(cfg1, oracle, gpu_ctx):
+    _, noisy = cfg1
+    prm = oracle.params_1(20.0)
+    refs = all_refs(176, 144, 10, 1, prm.k, prm.p)
+    import vbm3d_amd as V
+    nsx = _bm_compare(noisy, V.initialize_parameters_1(20.0), refs, oracle, gpu_ctx)
+    assert set(np.unique(nsx)) <= {1, 2, 4, 8}
+
+
+def test_bm_config1_step2_params(cfg1, oracle, gpu_ctx):
+    import vbm3d_amd as V
+    _, noisy = cfg1
+    prm = V.initialize_parameters_2(20.0)
+    refs = all_refs(176, 144, 10, 1, prm.k, prm.p)
+    _bm_compare(noisy, prm, refs, oracle, gpu_ctx)
+
+
+def test_bm_rgb_colour(small_rgb, oracle, gpu_ctx):
+    import vbm3d_amd as V
+    _, noisy = small_rgb
+    yuv = oracle.color(noisy, True)
+    for prm in (V.initialize_parameters_1(20.0), V.initialize_parameters_2(20.0),
+                V.initialize_parameters_2(40.0)):
+        refs = all_refs(96, 72, 7, 3, prm.k, prm.p)
+        _bm_compare(yuv, prm, refs, oracle, gpu_ctx)
+
+
+def test_bm_ties(oracle, gpu_ctx):
+    """Integer-valued clip with 3 levels: distances tie everywhere, so the
+    heap-select tie order of std::partial_sort decides."""
+    import vbm3d_amd as V
+    rng = np.random.default_rng(5)
+    vid = rng.integers(0, 3, (6, 1, 40, 48)).astype(np.float32)
+    for prm in (V.initialize_parameters_1(20.0), V.initialize_parameters_2(20.0)):
+        refs = all_refs(48, 40, 6, 1, prm.k, prm.p)
+        _bm_compare(vid, prm, refs, oracle, gpu_ctx)
+    flat = np.full((5, 1, 24, 24), 7.0, np.float32)
+    prm = V.initialize_parameters_1(20.0)
+    _bm_compare(flat, prm, all_refs(24, 24, 5, 1, prm.k, prm.p), oracle, gpu_ctx)
+
+
+def test_bm_with_flow(oracle, gpu_ctx):
+    import vbm3d_amd as V
+    _, noisy = synth.noisy_clip(64, 48, 6, 1, 20.0, 3)
+    rng = np.random.default_rng(9)
+    ff = rng.uniform(-2.6, 2.6, (5, 2, 48, 64)).astype(np.float32)
+    bf = rng.uniform(-2.6, 2.6, (5, 2, 48, 64)).astype(np.float32)
+    prm = V.initialize_parameters_1(20.0)
+    refs = all_refs(64, 48, 6, 1, prm.k, prm.p)
+    _bm_compare(noisy, prm, refs, oracle, gpu_ctx, ff, bf)
+
+
+@pytest.mark.parametrize("nb", [1, 4])
+def test_step1_sums_gray(cfg1, oracle, gpu_ctx, nb):
+    import vbm3d_amd as V
+    _, noisy = cfg1
+    prm = V.initialize_parameters_1(20.0)
+    gn, gd = V.step(1, 20.0, noisy, None, prm, True, nb, ctx=gpu_ctx)
+    on, od = oracle.step(1, 20.0, noisy, None, prm, True, nb)
+    assert_bits_equal(gd, od, "step-1 denominator")
+    assert_bits_equal(gn, on, "step-1 numerator")
+
+
+def test_step_sums_rgb(small_rgb, oracle, gpu_ctx):
+    import vbm3d_amd as V
+    _, noisy = small_rgb
+    yuv = oracle.color(noisy, True)
+    p1 = V.initialize_parameters_1(20.0)
+    gn, gd = V.step(1, 20.0, yuv, None, p1, True, 1, ctx=gpu_ctx)
+    on, od = oracle.step(1, 20.0, yuv, None, p1, True, 1)
+    assert_bits_equal(gd, od, "step-1 denominator")
+    assert_bits_equal(gn, on, "step-1 numerator")
+    basic = np.where(od == 0, yuv, on / np.where(od == 0, 1, od)).astype(np.float32)
+    p2 = V.initialize_parameters_2(20.0)
+    gn, gd = V.step(2, 20.0, yuv, basic, p2, True, 1, ctx=gpu_ctx)
+    on, od = oracle.step(2, 20.0, yuv, basic, p2, True, 1)
+    assert_bits_equal(gd, od, "step-2 denominator")
+    assert_bits_equal(gn, on, "step-2 numerator")
+
+
+def _run_both(noisy, sigma, oracle, ctx, nb=1, color=True, p1=None, p2=None):
+    import vbm3d_amd as V
+    p1 = p1 or V.initialize_parameters_1(sigma)
+    p2 = p2 or V.initialize_parameters_2(sigma)
+    nz = noisy.copy()
+    gb = np.zeros_like(nz)
+    gf = np.zeros_like(nz)
+    V.run_vbm3d(sigma, nz, None, None, gb, gf, p1, p2, color, nb, ctx=ctx)
+    ob, of, onz = oracle.run_vbm3d(sigma, noisy, p1, p2, color, nb)
+    assert_bits_equal(gb, ob, "basic")
+    assert_bits_equal(gf, of, "final")
+    assert_bits_equal(nz, onz, "noisy after the colour round trip")
+    return gb, gf
+
+
+def test_run_config1(cfg1, oracle, gpu_ctx):
+    clean, noisy = cfg1
+    gb, gf = _run_both(noisy, 20.0, oracle, gpu_ctx)
+    assert synth.psnr(clean, gf) > synth.psnr(clean, gb) > synth.psnr(clean, noisy) + 8
+
+
+@pytest.mark.parametrize("nb", [2, 8])
+def test_run_config1_threaded_grid(cfg1, oracle, gpu_ctx, nb):
+    _, noisy = cfg1
+    _run_both(noisy, 20.0, oracle, gpu_ctx, nb=nb)
+
+
+@pytest.mark.parametrize("sigma", [20.0, 40.0])
+def test_run_rgb(small_rgb, oracle, gpu_ctx, sigma):
+    clean, _ = small_rgb
+    rng = np.random.default_rng(int(sigma))
+    noisy = (clean + np.float32(sigma) * rng.standard_normal(clean.shape, dtype=np.float32)).astype(np.float32)
+    gb, gf = _run_both(noisy, sigma, oracle, gpu_ctx)
+    assert synth.psnr(clean, gf) > synth.psnr(clean, noisy) + 6
+
+
+def test_run_rgb_no_colour_and_hadamard(small_rgb, oracle, gpu_ctx):
+    import vbm3d_amd as V
+    _, noisy = small_rgb
+    p1 = V.initialize_parameters_1(20.0, T_3D=V.HADAMARD)
+    p2 = V.initialize_parameters_2(20.0, T_3D=V.HADAMARD)
+    _run_both(noisy, 20.0, oracle, gpu_ctx, color=False, p1=p1, p2=p2)
+    p1 = V.initialize_parameters_1(20.0, T_2D=V.DCT)
+    p2 = V.initialize_parameters_2(40.0, T_2D=V.BIOR)
+    _run_both(noisy, 20.0, oracle, gpu_ctx, p1=p1, p2=p2)
+
+
+def test_run_chunked_ring(cfg1, oracle, gpu_ctx, monkeypatch):
+    """A tiny estimate budget forces one-frame chunks and a ring of
+    2 Nf + 1 frames; the result must not change."""
+    import vbm3d_amd as V
+    _, noisy = cfg1
+    monkeypatch.setenv("VBM3D_EST_BUDGET_MB", "1")
+    with V.Context(0) as ctx:
+        _run_both(noisy, 20.0, oracle, ctx)

@@ -1,0 +1,13 @@
+"""vbm3d_amd -- MI355X-native VBM3D collaborative filter.
+
+The denoising path of tehret/vbm3d (run_vbm3d, src/vbm3d.cpp:87-376) on
+hand-written gfx950 HIP kernels behind a C ABI (include/vbm3d_hip.h,
+libvbm3d_hip.so).  The Python API mirrors the reference's interface.
+"""
+from .api import (BIOR, DCT, HAAR, HADAMARD, NONE, Context, Parameters, VBM3DError, block_match,
+                  denoise, grid, initialize_parameters_1, initialize_parameters_2, lib, run_device,
+                  run_vbm3d, step)
+
+__all__ = ["BIOR", "DCT", "HAAR", "HADAMARD", "NONE", "Context", "Parameters", "VBM3DError",
+           "block_match", "denoise", "grid", "initialize_parameters_1", "initialize_parameters_2",
+           "lib", "run_device", "run_vbm3d", "step"]

@@ -1,0 +1,282 @@
+"""Python mirror of the reference's denoising interface over the C ABI.
+
+Names, argument meaning and error behaviour follow the reference:
+``Parameters`` (src/parameters.h:21-58), ``initialize_parameters_1/_2``
+(src/main.cpp:35-207) and ``run_vbm3d`` (src/vbm3d.cpp:87-376), with
+numpy arrays in the reference's planar [F][C][H][W] layout in place of
+``Video<float>``.  Everything runs through libvbm3d_hip.so on the GPU;
+there is no CPU fallback: a missing library or GPU raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import threading
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libvbm3d_hip.so")
+
+NONE = -1
+DCT, BIOR, HADAMARD, HAAR = 0, 1, 2, 3
+
+
+class VBM3DError(RuntimeError):
+    """Error returned through the C ABI (code < 0)."""
+
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"vbm3d error {code}: {msg}")
+        self.code = code
+
+
+class Parameters(C.Structure):
+    """struct Parameters (src/parameters.h:21-58), field for field."""
+    _fields_ = [("k", C.c_uint), ("kt", C.c_uint), ("p", C.c_uint), ("N", C.c_uint),
+                ("tau", C.c_float), ("d", C.c_float),
+                ("Nf", C.c_uint), ("Ns", C.c_uint), ("Npr", C.c_uint), ("Nb", C.c_uint),
+                ("T_2D", C.c_uint), ("T_3D", C.c_uint), ("lambda3D", C.c_float),
+                ("mc", C.c_int), ("n", C.c_int)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+    def copy(self):
+        o = Parameters()
+        C.memmove(C.byref(o), C.byref(self), C.sizeof(Parameters))
+        return o
+
+
+_lib = None
+_lock = threading.Lock()
+
+
+def lib():
+    """Load libvbm3d_hip.so (built by vbm3d_amd.build); raises if absent."""
+    global _lib
+    with _lock:
+        if _lib is None:
+            if not os.path.exists(LIB_PATH):
+                raise ImportError(f"{LIB_PATH} is missing: run `python -m vbm3d_amd.build` first")
+            L = C.CDLL(LIB_PATH)
+            fp, up, pp = C.POINTER(C.c_float), C.POINTER(C.c_uint32), C.POINTER(Parameters)
+            L.vbm3d_create.argtypes = [C.c_int]
+            L.vbm3d_create.restype = C.c_void_p
+            L.vbm3d_destroy.argtypes = [C.c_void_p]
+            L.vbm3d_last_error.restype = C.c_char_p
+            L.vbm3d_default_params_1.argtypes = [pp] + [C.c_int] * 9 + [C.c_float, C.c_float, C.c_uint, C.c_uint, C.c_float]
+            L.vbm3d_default_params_2.argtypes = [pp] + [C.c_int] * 9 + [C.c_float, C.c_uint, C.c_uint, C.c_float]
+            L.vbm3d_run.argtypes = [C.c_void_p, C.c_float, fp, fp, fp, C.c_uint, C.c_uint, C.c_uint, C.c_uint,
+                                    pp, pp, C.c_int, C.c_int, fp, fp]
+            L.vbm3d_run_device.argtypes = [C.c_void_p, C.c_float, C.c_void_p, C.c_void_p, C.c_void_p,
+                                           C.c_uint, C.c_uint, C.c_uint, C.c_uint, pp, pp, C.c_int, C.c_int,
+                                           C.c_void_p, C.c_void_p, C.c_void_p]
+            L.vbm3d_block_match.argtypes = [C.c_void_p, fp, C.c_uint, C.c_uint, C.c_uint, C.c_uint, fp, fp, pp,
+                                            C.c_long, up, up, fp, up, up]
+            L.vbm3d_step.argtypes = [C.c_void_p, C.c_int, C.c_float, fp, fp, fp, fp, C.c_uint, C.c_uint,
+                                     C.c_uint, C.c_uint, pp, C.c_int, C.c_int, fp, fp]
+            L.vbm3d_grid.argtypes = [C.c_uint, C.c_uint, C.c_uint, C.c_uint, C.c_int, C.c_int, up, up, up]
+            L.vbm3d_grid.restype = C.c_long
+            L.vbm3d_selftest_partial_sort.argtypes = [fp, up, C.c_uint, C.c_uint]
+            L.vbm3d_profile.argtypes = [C.c_void_p, C.c_int]
+            L.vbm3d_profile_read.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double),
+                                             C.POINTER(C.c_long), C.POINTER(C.c_double)]
+            _lib = L
+    return _lib
+
+
+def _check(rc: int):
+    if rc != 0:
+        raise VBM3DError(rc, lib().vbm3d_last_error().decode())
+
+
+def _f(a):
+    return None if a is None else a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def _u(a):
+    return a.ctypes.data_as(C.POINTER(C.c_uint32))
+
+
+def initialize_parameters_1(sigma, k=NONE, kt=NONE, Nf=NONE, Ns=NONE, Npr=NONE, Nb=NONE, p=NONE,
+                            N=NONE, d=NONE, tau=NONE, lambda3D=NONE, T_2D=NONE, T_3D=NONE) -> Parameters:
+    """Step-1 defaults (initializeParameters_1, src/main.cpp:35-125)."""
+    o = Parameters()
+    lib().vbm3d_default_params_1(C.byref(o), k, kt, Nf, Ns, Npr, Nb, p, N, d, float(tau), float(lambda3D),
+                                 T_2D & 0xFFFFFFFF, T_3D & 0xFFFFFFFF, float(sigma))
+    return o
+
+
+def initialize_parameters_2(sigma, k=NONE, kt=NONE, Nf=NONE, Ns=NONE, Npr=NONE, Nb=NONE, p=NONE,
+                            N=NONE, d=NONE, tau=NONE, T_2D=NONE, T_3D=NONE) -> Parameters:
+    """Step-2 defaults (initializeParameters_2, src/main.cpp:127-207)."""
+    o = Parameters()
+    lib().vbm3d_default_params_2(C.byref(o), k, kt, Nf, Ns, Npr, Nb, p, N, d, float(tau),
+                                 T_2D & 0xFFFFFFFF, T_3D & 0xFFFFFFFF, float(sigma))
+    return o
+
+
+class Context:
+    """A device context (owns device buffers and a HIP stream)."""
+
+    def __init__(self, device: int = 0):
+        self.h = lib().vbm3d_create(device)
+        if not self.h:
+            raise VBM3DError(-2, lib().vbm3d_last_error().decode())
+
+    def close(self):
+        if self.h:
+            lib().vbm3d_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_default_ctx = None
+
+
+def default_context() -> Context:
+    global _default_ctx
+    if _default_ctx is None:
+        _default_ctx = Context(0)
+    return _default_ctx
+
+
+def _video(a, name):
+    if not isinstance(a, np.ndarray) or a.dtype != np.float32 or a.ndim != 4 or not a.flags.c_contiguous:
+        raise TypeError(f"{name} must be a C-contiguous float32 array [F][C][H][W]")
+    return a
+
+
+def run_vbm3d(sigma, vid_noisy, fflow, bflow, vid_basic, vid_denoised, prms_1, prms_2,
+              color_space=True, ref_grid_nb=1, ctx: Context | None = None) -> int:
+    """run_vbm3d (src/vbm3d.cpp:87-376).  vid_noisy is mutated by the
+    colour round trip like the reference; vid_basic / vid_denoised are
+    filled in place (vid_basic is read when prms_1.k == 0).  Returns 0
+    (EXIT_SUCCESS); raises VBM3DError on failure."""
+    _video(vid_noisy, "vid_noisy")
+    _video(vid_basic, "vid_basic")
+    _video(vid_denoised, "vid_denoised")
+    F, Cc, H, W = vid_noisy.shape
+    if vid_basic.shape != vid_noisy.shape or vid_denoised.shape != vid_noisy.shape:
+        raise ValueError("vid_basic / vid_denoised must have vid_noisy's shape")
+    ff = None if fflow is None else np.ascontiguousarray(fflow, np.float32)
+    bf = None if bflow is None else np.ascontiguousarray(bflow, np.float32)
+    ctx = ctx or default_context()
+    _check(lib().vbm3d_run(ctx.h, float(sigma), _f(vid_noisy), _f(ff), _f(bf), W, H, F, Cc,
+                           C.byref(prms_1), C.byref(prms_2), int(bool(color_space)), int(ref_grid_nb),
+                           _f(vid_basic), _f(vid_denoised)))
+    return 0
+
+
+def denoise(noisy, sigma, color_space=True, ref_grid_nb=1, prms_1=None, prms_2=None, ctx=None):
+    """Convenience: returns (basic, final) for a float32 [F][C][H][W] clip
+    without touching the input."""
+    nz = np.array(noisy, np.float32, copy=True, order="C")
+    basic = np.zeros_like(nz)
+    final = np.zeros_like(nz)
+    p1 = prms_1 or initialize_parameters_1(sigma)
+    p2 = prms_2 or initialize_parameters_2(sigma)
+    run_vbm3d(sigma, nz, None, None, basic, final, p1, p2, color_space, ref_grid_nb, ctx)
+    return basic, final
+
+
+def run_device(sigma, noisy_t, basic_t, final_t, prms_1, prms_2, color_space=True, ref_grid_nb=1,
+               fflow_t=None, bflow_t=None, stream=None, ctx: Context | None = None) -> int:
+    """Device-resident run on torch CUDA tensors (float32, contiguous,
+    [F][C][H][W]); noisy_t is mutated like the reference's vid_noisy."""
+    F, Cc, H, W = noisy_t.shape
+    ctx = ctx or default_context()
+    sp = None
+    if stream is not None:
+        sp = C.c_void_p(int(stream.cuda_stream))
+    _check(lib().vbm3d_run_device(ctx.h, float(sigma), C.c_void_p(noisy_t.data_ptr()),
+                                  None if fflow_t is None else C.c_void_p(fflow_t.data_ptr()),
+                                  None if bflow_t is None else C.c_void_p(bflow_t.data_ptr()),
+                                  W, H, F, Cc, C.byref(prms_1), C.byref(prms_2), int(bool(color_space)),
+                                  int(ref_grid_nb), C.c_void_p(basic_t.data_ptr()),
+                                  C.c_void_p(final_t.data_ptr()), sp))
+    return 0
+
+
+def block_match(vid, prm, ref_idx, fflow=None, bflow=None, ctx=None):
+    """computeSimilarPatches (src/vbm3d.cpp:1082-1197) for a list of
+    reference indices t*W*H*C + y*W + x.  Returns (idx[n,N], dist[n,N],
+    nsim[n], nsx[n])."""
+    vid = np.ascontiguousarray(vid, np.float32)
+    F, Cc, H, W = vid.shape
+    ref_idx = np.ascontiguousarray(ref_idx, np.uint32)
+    n = len(ref_idx)
+    N = prm.N
+    idx = np.empty((n, N), np.uint32)
+    dist = np.empty((n, N), np.float32)
+    nsim = np.empty(n, np.uint32)
+    nsx = np.empty(n, np.uint32)
+    ff = None if fflow is None else np.ascontiguousarray(fflow, np.float32)
+    bf = None if bflow is None else np.ascontiguousarray(bflow, np.float32)
+    ctx = ctx or default_context()
+    _check(lib().vbm3d_block_match(ctx.h, _f(vid), W, H, F, Cc, _f(ff), _f(bf), C.byref(prm), n,
+                                   _u(ref_idx), _u(idx), _f(dist), _u(nsim), _u(nsx)))
+    return idx, dist, nsim, nsx
+
+
+def step(stepno, sigma, noisy, basic, prm, color_space=True, ref_grid_nb=1, fflow=None, bflow=None,
+         ctx=None):
+    """One step on a colour-transformed video; returns (num, den)."""
+    nz = np.ascontiguousarray(noisy, np.float32)
+    F, Cc, H, W = nz.shape
+    bs = None if basic is None else np.ascontiguousarray(basic, np.float32)
+    ff = None if fflow is None else np.ascontiguousarray(fflow, np.float32)
+    bf = None if bflow is None else np.ascontiguousarray(bflow, np.float32)
+    num = np.empty_like(nz)
+    den = np.empty_like(nz)
+    ctx = ctx or default_context()
+    _check(lib().vbm3d_step(ctx.h, stepno, float(sigma), _f(nz), _f(bs), _f(ff), _f(bf), W, H, F, Cc,
+                            C.byref(prm), int(bool(color_space)), int(ref_grid_nb), _f(num), _f(den)))
+    return num, den
+
+
+def grid(W, H, k, p, n=16, nb=1):
+    """Reference-patch grid of one frame: (xs, ys, crop) in processing order."""
+    cnt = lib().vbm3d_grid(W, H, k, p, n, nb, None, None, None)
+    if cnt < 0:
+        raise ValueError("invalid grid")
+    xs = np.empty(cnt, np.uint32)
+    ys = np.empty(cnt, np.uint32)
+    cr = np.empty(cnt, np.uint32)
+    lib().vbm3d_grid(W, H, k, p, n, nb, _u(xs), _u(ys), _u(cr))
+    return xs, ys, cr
+
+
+def selftest_partial_sort(dist, idx, mid):
+    d = np.array(dist, np.float32, copy=True)
+    i = np.array(idx, np.uint32, copy=True)
+    lib().vbm3d_selftest_partial_sort(_f(d), _u(i), len(d), mid)
+    return d, i
+
+
+KERNELS = {"block_match": 0, "group_filter": 1, "aggregate": 2, "colour": 3, "bm_pixels": 4}
+
+
+def profile(enable: bool, ctx=None):
+    """Reset and enable (or disable) kernel timing on a context."""
+    ctx = ctx or default_context()
+    _check(lib().vbm3d_profile(ctx.h, int(bool(enable))))
+
+
+def profile_read(kernel: str, ctx=None):
+    """(total_ms, launches, work) for one kernel class since profile(True)."""
+    ctx = ctx or default_context()
+    ms, n, w = C.c_double(), C.c_long(), C.c_double()
+    _check(lib().vbm3d_profile_read(ctx.h, KERNELS[kernel], C.byref(ms), C.byref(n), C.byref(w)))
+    return ms.value, n.value, w.value

@@ -1,0 +1,735 @@
+// capi.hip -- host driver and C ABI of libvbm3d_hip.so.
+//
+// The driver replaces run_vbm3d's thread tiling and per-row loops
+// (src/vbm3d.cpp:87-376, 527-653, 759-927) with a frame-chunked pipeline
+// on one HIP stream:
+//     for each chunk of reference frames:
+//         k_bm        -> group indices and sizes (ring slot per reference)
+//         k_filter    -> filtered group estimates and weights
+//         k_aggregate -> output frames whose contributors are all done
+// Per-reference results live in a ring of S + 2 Nf frames, so device
+// memory does not grow with the clip length.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/vbm3d_hip.h"
+#include "kernels.h"
+#include "select.h"
+
+using namespace vb;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                  \
+    do {                                                                               \
+        hipError_t e_ = (expr);                                                        \
+        if (e_ != hipSuccess)                                                          \
+            return fail(VBM3D_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+};
+
+hipError_t ensure(DevBuf& b, size_t bytes) {
+    if (bytes <= b.cap) return hipSuccess;
+    if (b.p) {
+        hipError_t e = hipFree(b.p);
+        if (e != hipSuccess) return e;
+        b.p = nullptr;
+        b.cap = 0;
+    }
+    hipError_t e = hipMalloc(&b.p, bytes);
+    if (e == hipSuccess) b.cap = bytes;
+    return e;
+}
+
+// ---------------------------------------------------------------------
+// Reference-patch grid (src/Utilities/Utilities.cpp:152-173, 400-415;
+// src/Utilities/LibVideoT.hpp:903-961; src/vbm3d.cpp:471-489, 700-719).
+// ---------------------------------------------------------------------
+struct GridDef {
+    std::vector<std::vector<unsigned>> rows, cols;  // per crop, absolute
+};
+
+bool axis(std::vector<unsigned>& out, long b, long e, unsigned step) {
+    out.clear();
+    if (b < 0 || e < b || step == 0) return false;
+    long v = b;
+    for (; v <= e; v += step) out.push_back((unsigned)v);
+    if ((long)out.back() < e) out.push_back((unsigned)e);
+    return true;
+}
+
+bool build_grid(GridDef& g, unsigned W, unsigned H, unsigned k, unsigned p, int n, int nb) {
+    g.rows.clear();
+    g.cols.clear();
+    if (W < k || H < k || p == 0) return false;
+    if (nb == 1) {
+        std::vector<unsigned> r, c;
+        if (!axis(r, 0, (long)H - k, p) || !axis(c, 0, (long)W - k, p)) return false;
+        g.rows.push_back(r);
+        g.cols.push_back(c);
+        return true;
+    }
+    if (nb < 2 || nb > 32 || (nb & (nb - 1))) return false;
+    // tiles: the smallest divisor >= 2 and its cofactor, wider than tall
+    unsigned b = 2;
+    while ((unsigned)nb % b) b++;
+    unsigned a = nb / b;
+    if (b > a) std::swap(a, b);
+    const unsigned nW = a, nH = b;
+    const int wT = (int)std::ceil(float(W) / float(nW));
+    const int hT = (int)std::ceil(float(H) / float(nH));
+    const int border = 2 * n;
+    for (int ty = 0; ty < (int)nH; ty++)
+        for (int tx = 0; tx < (int)nW; tx++) {
+            const int ox = std::max(0, tx * wT - border), oy = std::max(0, ty * hT - border);
+            const int ex = std::min((int)W, (tx + 1) * wT + border);
+            const int ey = std::min((int)H, (ty + 1) * hT + border);
+            std::vector<unsigned> r, c;
+            const long rb = oy == 0 ? 0 : n, cb = ox == 0 ? 0 : n;
+            const long re = ey == (int)H ? (long)ey - oy - k : (long)ey - oy - k - n;
+            const long ce = ex == (int)W ? (long)ex - ox - k : (long)ex - ox - k - n;
+            if (!axis(r, rb, re, p) || !axis(c, cb, ce, p)) return false;
+            for (auto& v : r) v += oy;
+            for (auto& v : c) v += ox;
+            g.rows.push_back(r);
+            g.cols.push_back(c);
+        }
+    return true;
+}
+
+// Per-channel noise (estimate_sigma, src/Utilities/Utilities.cpp:282-333)
+// as called by the reference with a bool colour flag (src/vbm3d.cpp:467):
+// true -> YCbCr row, false -> YUV row.
+void sigma_rows(float sigma, unsigned C, bool cs, float* out) {
+    if (C == 1) {
+        out[0] = sigma;
+        return;
+    }
+    out[0] = sqrtf(0.299f * 0.299f + 0.587f * 0.587f + 0.114f * 0.114f) * sigma;
+    if (cs) {
+        out[1] = sqrtf(0.169f * 0.169f + 0.331f * 0.331f + 0.500f * 0.500f) * sigma;
+        out[2] = sqrtf(0.500f * 0.500f + 0.419f * 0.419f + 0.081f * 0.081f) * sigma;
+    } else {
+        out[1] = sqrtf(0.14713f * 0.14713f + 0.28886f * 0.28886f + 0.436f * 0.436f) * sigma;
+        out[2] = sqrtf(0.615f * 0.615f + 0.51498f * 0.51498f + 0.10001f * 0.10001f) * sigma;
+    }
+}
+
+// Bior1.5 MATLAB basis (src/lib_transforms.cpp:141-148, 281-288), the
+// same decimal values rounded to float.
+const double kBF[8][8] = {
+    {0.353553390593274, 0.353553390593274, 0.353553390593274, 0.353553390593274, 0.353553390593274, 0.353553390593274, 0.353553390593274, 0.353553390593274},
+    {0.219417649252501, 0.449283757993216, 0.449283757993216, 0.219417649252501, -0.219417649252501, -0.449283757993216, -0.449283757993216, -0.219417649252501},
+    {0.569359398342846, 0.402347308162278, -0.402347308162278, -0.569359398342846, -0.083506045090284, 0.083506045090284, -0.083506045090284, 0.083506045090284},
+    {-0.083506045090284, 0.083506045090284, -0.083506045090284, 0.083506045090284, 0.569359398342846, 0.402347308162278, -0.402347308162278, -0.569359398342846},
+    {0.707106781186547, -0.707106781186547, 0, 0, 0, 0, 0, 0},
+    {0, 0, 0.707106781186547, -0.707106781186547, 0, 0, 0, 0},
+    {0, 0, 0, 0, 0.707106781186547, -0.707106781186547, 0, 0},
+    {0, 0, 0, 0, 0, 0, 0.707106781186547, -0.707106781186547}};
+const double kBI[8][8] = {
+    {0.353553390593274, 0.353553390593274, 0.5, 0.0, 0.707106781186547, -0.121533978016438, 0.0, 0.121533978016438},
+    {0.353553390593274, 0.353553390593274, 0.5, -0.0, -0.707106781186547, -0.121533978016438, 0.0, 0.121533978016438},
+    {0.353553390593274, 0.353553390593274, -0.5, -0.0, 0.121533978016438, 0.707106781186548, -0.121533978016438, 0.0},
+    {0.353553390593274, 0.353553390593274, -0.5, 0.0, 0.121533978016438, -0.707106781186548, -0.121533978016438, 0.0},
+    {0.353553390593274, -0.353553390593274, 0.0, 0.5, 0.0, 0.121533978016438, 0.707106781186547, -0.121533978016438},
+    {0.353553390593274, -0.353553390593274, 0.0, 0.5, 0.0, 0.121533978016438, -0.707106781186547, -0.121533978016438},
+    {0.353553390593274, -0.353553390593274, 0.0, -0.5, -0.121533978016438, 0.0, 0.121533978016438, 0.707106781186547},
+    {0.353553390593274, -0.353553390593274, 0.0, -0.5, -0.121533978016438, 0.0, 0.121533978016438, -0.707106781186547}};
+
+// Kaiser(beta = 2) quarter tables (src/vbm3d.cpp:1755-1841).
+float kaiser_at(unsigned k, unsigned a, unsigned b) {
+    static const float q4[2][2] = {{0.1924f, 0.4055f}, {0.4055f, 0.8544f}};
+    static const float q6[3][3] = {{0.1924f, 0.3368f, 0.4265f}, {0.3368f, 0.5893f, 0.7464f}, {0.4265f, 0.7464f, 0.9454f}};
+    static const float q7[4][4] = {{0.1924f, 0.3151f, 0.4055f, 0.4387f}, {0.3151f, 0.5161f, 0.6640f, 0.7184f},
+                                   {0.4055f, 0.6640f, 0.8544f, 0.9243f}, {0.4387f, 0.7184f, 0.9243f, 1.0000f}};
+    static const float q8[4][4] = {{0.1924f, 0.2989f, 0.3846f, 0.4325f}, {0.2989f, 0.4642f, 0.5974f, 0.6717f},
+                                   {0.3846f, 0.5974f, 0.7688f, 0.8644f}, {0.4325f, 0.6717f, 0.8644f, 0.9718f}};
+    if (k == 4) return q4[a][b];
+    if (k == 6) return q6[a][b];
+    if (k == 7) return q7[a][b];
+    if (k == 8) return q8[a][b];
+    return 1.0f;
+}
+
+void fill_tabs(Tabs& t, unsigned k) {
+    std::memset(&t, 0, sizeof t);
+    for (int a = 0; a < 8; a++)
+        for (int b = 0; b < 8; b++) {
+            t.biorF[a * 8 + b] = (float)kBF[a][b];
+            t.biorI[a * 8 + b] = (float)kBI[a][b];
+        }
+    // REDFT10 / REDFT01 closed forms, double then one rounding to float
+    for (unsigned v = 0; v < k; v++)
+        for (unsigned q = 0; q < k; q++) {
+            t.dctF[v * k + q] = (float)(2.0 * std::cos(M_PI * (2.0 * q + 1.0) * v / (2.0 * k)));
+            t.dctI[q * k + v] = v == 0 ? 1.0f : (float)(2.0 * std::cos(M_PI * v * (2.0 * q + 1.0) / (2.0 * k)));
+        }
+    // coef_norm / coef_norm_inv (src/vbm3d.cpp:1843-1863)
+    const float coef = 0.5f / ((float)k);
+    for (unsigned i = 0; i < k; i++)
+        for (unsigned j = 0; j < k; j++) {
+            if (i == 0 && j == 0) {
+                t.nrm[i * k + j] = 0.5f * coef;
+                t.inv[i * k + j] = 2.0f;
+            } else if (i * j == 0) {
+                t.nrm[i * k + j] = (float)(0.7071067811865475 * coef);
+                t.inv[i * k + j] = (float)1.414213562373095;
+            } else {
+                t.nrm[i * k + j] = 1.0f * coef;
+                t.inv[i * k + j] = 1.0f;
+            }
+        }
+    for (unsigned p = 0; p < k; p++)
+        for (unsigned q = 0; q < k; q++)
+            t.kaiser[p * k + q] = kaiser_at(k, std::min(p, k - 1 - p), std::min(q, k - 1 - q));
+    t.dct_out = 1.0f / (float)(k * 2);
+}
+
+// What the kernels implement; everything else is rejected with a message.
+int check(const vbm3d_params& p, unsigned W, unsigned H, unsigned F, unsigned C, int step) {
+    char b[160];
+    auto no = [&](int code, const char* m) {
+        std::snprintf(b, sizeof b, "step %d: %s", step, m);
+        return fail(code, b);
+    };
+    if (C != 1 && C != 3) return no(VBM3D_ERR_ARG, "only 1 or 3 channels are supported");
+    if (W == 0 || H == 0 || F == 0) return no(VBM3D_ERR_ARG, "empty video");
+    if (W > 65535 || H > 65535) return no(VBM3D_ERR_ARG, "frame dimensions above 65535");
+    if ((unsigned long long)W * H * C * F >= (1ull << 32))
+        return no(VBM3D_ERR_ARG, "video too large for 32-bit patch indices");
+    if (p.k < 4 || p.k > 8) return no(VBM3D_ERR_UNSUPPORTED, "patch size k must be in [4, 8]");
+    if (W < p.k || H < p.k) return no(VBM3D_ERR_ARG, "frame smaller than a patch");
+    if (p.kt != 1) return no(VBM3D_ERR_UNSUPPORTED, "only kt == 1 is supported");
+    if (p.N == 0 || p.N > (unsigned)kMaxN || (p.N & (p.N - 1)))
+        return no(VBM3D_ERR_UNSUPPORTED, "N must be a power of two <= 16");
+    if (p.Nb == 0 || p.Nb > 8) return no(VBM3D_ERR_UNSUPPORTED, "Nb must be in [1, 8]");
+    if (p.Ns == 0 || p.Ns > (unsigned)kMaxWin || p.Npr == 0 || p.Npr > (unsigned)kMaxWin)
+        return no(VBM3D_ERR_UNSUPPORTED, "Ns and Npr must be in [1, 8]");
+    if (p.Nb * p.Npr * p.Npr > (unsigned)kMaxSlots) return no(VBM3D_ERR_UNSUPPORTED, "Nb * Npr^2 > 512");
+    if (p.Nb * (2 * p.Nf + 1) > (unsigned)kMaxBest) return no(VBM3D_ERR_UNSUPPORTED, "Nb * (2 Nf + 1) > 64");
+    if (p.p == 0) return no(VBM3D_ERR_ARG, "step p must be > 0");
+    if (p.T_2D != VBM3D_DCT && p.T_2D != VBM3D_BIOR) return no(VBM3D_ERR_ARG, "T_2D must be DCT (0) or BIOR (1)");
+    if (p.T_2D == VBM3D_BIOR && p.k != 8) return no(VBM3D_ERR_ARG, "BIOR needs k == 8");
+    // each search window must yield Nb candidates (src/vbm3d.cpp:1115-1119)
+    for (unsigned s : {p.Ns, p.Npr}) {
+        const unsigned h = (s - 1) / 2 + 1;
+        if (std::min(W - p.k + 1, h) * std::min(H - p.k + 1, h) < p.Nb)
+            return no(VBM3D_ERR_UNSUPPORTED, "search window smaller than Nb");
+    }
+    return VBM3D_OK;
+}
+
+}  // namespace
+
+struct ProfEvent {
+    int kernel;
+    hipEvent_t a, b;
+};
+
+struct vbm3d_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    DevBuf noisy, basic, den, ff, bf, num_raw, den_raw;  // host-buffer entry points
+    DevBuf idx, nsx, nsim, dist, est, wgt, tabs, refxy, meta, gpos, rlist, scal, stats;
+    size_t est_budget = (size_t)32 << 30;
+    // kernel profiling (vbm3d_profile / vbm3d_profile_read)
+    bool prof = false;
+    std::vector<ProfEvent> events;
+    double refs_filtered = 0;
+    double contributions = 0;
+};
+
+namespace {
+
+struct StepIO {
+    int step;
+    float sigma;
+    const float* noisy;    // colour-transformed
+    const float* basic;    // step 2
+    const float* ff;
+    const float* bf;
+    unsigned W, H, F, C;
+    const vbm3d_params* p;
+    bool cs;
+    int nb;
+    float* out;            // may be null
+    const float* fallback;
+    float* num_raw;        // may be null
+    float* den_raw;
+    unsigned maxflow;
+};
+
+// Brackets one launch with events when profiling is on.
+template <class F>
+hipError_t timed(vbm3d_ctx* ctx, int kernel, hipStream_t s, F&& launch) {
+    if (!ctx->prof) return launch();
+    ProfEvent ev{kernel, nullptr, nullptr};
+    hipError_t e = hipEventCreate(&ev.a);
+    if (e == hipSuccess) e = hipEventCreate(&ev.b);
+    if (e == hipSuccess) e = hipEventRecord(ev.a, s);
+    if (e == hipSuccess) e = launch();
+    if (e == hipSuccess) e = hipEventRecord(ev.b, s);
+    ctx->events.push_back(ev);
+    return e;
+}
+
+int run_step(vbm3d_ctx* ctx, const StepIO& io, hipStream_t s) {
+    const vbm3d_params& p = *io.p;
+    int rc = check(p, io.W, io.H, io.F, io.C, io.step);
+    if (rc) return rc;
+    GridDef g;
+    if (!build_grid(g, io.W, io.H, p.k, p.p, p.n, io.nb))
+        return fail(VBM3D_ERR_ARG, "invalid reference grid for these sizes (ref_grid_nb, p, n)");
+    // frame-local ref list in (crop, row, col) order, crop metadata
+    std::vector<uint32_t> refxy;
+    std::vector<int> meta;
+    std::vector<uint16_t> gpos;
+    for (size_t c = 0; c < g.rows.size(); c++) {
+        meta.push_back((int)refxy.size());
+        meta.push_back((int)g.rows[c].size());
+        meta.push_back((int)g.cols[c].size());
+        meta.push_back((int)gpos.size());
+        for (unsigned v : g.rows[c]) gpos.push_back((uint16_t)v);
+        meta.push_back((int)gpos.size());
+        for (unsigned v : g.cols[c]) gpos.push_back((uint16_t)v);
+        for (unsigned y : g.rows[c])
+            for (unsigned x : g.cols[c]) refxy.push_back(x | (y << 16));
+    }
+    const unsigned nrf = (unsigned)refxy.size();
+    const unsigned K = p.k, K2 = K * K, C = io.C, N = p.N;
+    const unsigned Fr = io.F - p.kt + 1;
+    // chunk size from the estimate budget; ring holds S + 2 Nf frames
+    const size_t per_frame = (size_t)nrf * C * N * K2 * sizeof(float);
+    size_t cap_frames = std::max<size_t>(1, ctx->est_budget / std::max<size_t>(per_frame, 1));
+    unsigned S = (unsigned)std::max<long>(1, (long)cap_frames - 2 * (long)p.Nf);
+    unsigned ring = std::min<unsigned>(Fr, S + 2 * p.Nf);
+    if (S >= Fr) { S = Fr; ring = Fr; }
+    const size_t slots = (size_t)ring * nrf;
+    if ((slots * kMaxN) >> 32) return fail(VBM3D_ERR_UNSUPPORTED, "too many reference patches in flight");
+
+    Tabs tabs;
+    fill_tabs(tabs, K);
+    HIP_TRY(ensure(ctx->idx, slots * N * 4));
+    HIP_TRY(ensure(ctx->nsx, slots * 4));
+    HIP_TRY(ensure(ctx->est, slots * C * N * K2 * 4));
+    HIP_TRY(ensure(ctx->wgt, slots * C * 4));
+    HIP_TRY(ensure(ctx->tabs, sizeof(Tabs)));
+    HIP_TRY(ensure(ctx->refxy, refxy.size() * 4));
+    HIP_TRY(ensure(ctx->meta, meta.size() * 4));
+    HIP_TRY(ensure(ctx->gpos, gpos.size() * 2));
+    HIP_TRY(hipMemcpyAsync(ctx->tabs.p, &tabs, sizeof tabs, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(ctx->refxy.p, refxy.data(), refxy.size() * 4, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(ctx->meta.p, meta.data(), meta.size() * 4, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(ctx->gpos.p, gpos.data(), gpos.size() * 2, hipMemcpyHostToDevice, s));
+
+    float sig[3] = {0, 0, 0};
+    sigma_rows(io.sigma, C, io.cs, sig);
+
+    BMArgs bm{};
+    bm.vid = io.step == 1 ? io.noisy : io.basic;
+    bm.ff = io.ff;
+    bm.bf = io.bf;
+    bm.W = io.W; bm.H = io.H; bm.F = io.F; bm.C = C;
+    bm.wh = io.W * io.H; bm.whc = bm.wh * C;
+    bm.nrf = nrf;
+    bm.ref_xy = (const uint32_t*)ctx->refxy.p;
+    bm.ref_list = nullptr;
+    bm.ring = ring;
+    bm.Ns = p.Ns; bm.Npr = p.Npr; bm.Nb = p.Nb; bm.Nf = p.Nf; bm.N = p.N; bm.kt = p.kt;
+    bm.d = p.d; bm.tau = p.tau;
+    bm.idx_out = (uint32_t*)ctx->idx.p;
+    bm.dist_out = nullptr;
+    bm.nsim_out = nullptr;
+    bm.nsx_out = (uint32_t*)ctx->nsx.p;
+    bm.stats = ctx->prof ? (unsigned long long*)ctx->stats.p : nullptr;
+
+    FArgs fa{};
+    fa.noisy = io.noisy;
+    fa.basic = io.basic;
+    fa.W = io.W; fa.H = io.H; fa.F = io.F; fa.C = C; fa.wh = bm.wh; fa.whc = bm.whc;
+    fa.nrf = nrf; fa.ring = ring;
+    fa.idx = bm.idx_out; fa.nsx = bm.nsx_out;
+    fa.N = p.N; fa.T2D = p.T_2D; fa.T3D = p.T_3D; fa.lambda = p.lambda3D;
+    for (int c = 0; c < 3; c++) fa.sig[c] = sig[c];
+    fa.tabs = (const Tabs*)ctx->tabs.p;
+    fa.est = (float*)ctx->est.p;
+    fa.wgt = (float*)ctx->wgt.p;
+
+    AArgs ag{};
+    ag.W = io.W; ag.H = io.H; ag.F = io.F; ag.C = C; ag.wh = bm.wh; ag.whc = bm.whc;
+    ag.K = K; ag.N = N; ag.Nf = p.Nf; ag.kt = p.kt;
+    ag.nrf = nrf; ag.ring = ring; ag.Fr = Fr;
+    ag.ncrop = (int)g.rows.size();
+    ag.crop_meta = (const int*)ctx->meta.p;
+    ag.grid_pos = (const uint16_t*)ctx->gpos.p;
+    ag.idx = bm.idx_out; ag.nsx = bm.nsx_out;
+    ag.est = fa.est; ag.wgt = fa.wgt; ag.tabs = fa.tabs;
+    ag.R = (int)((p.Ns - 1) / 2 + p.Nf * ((p.Npr - 1) / 2 + io.maxflow));
+    ag.fallback = io.fallback;
+    ag.out = io.out;
+    ag.num_out = io.num_raw;
+    ag.den_out = io.den_raw;
+
+    unsigned t_done = 0;
+    for (unsigned a = 0; a < Fr; a += S) {
+        const unsigned e = std::min(Fr, a + S);
+        bm.r_begin = fa.r_begin = (long)a * nrf;
+        bm.r_end = fa.r_end = (long)e * nrf;
+        HIP_TRY(timed(ctx, 0, s, [&] { return launch_bm((int)K, bm, s); }));
+        HIP_TRY(timed(ctx, 1, s, [&] { return launch_filter((int)K, io.step, fa, s); }));
+        if (ctx->prof) ctx->refs_filtered += (double)(bm.r_end - bm.r_begin);
+        const unsigned ready = e >= Fr ? io.F : (e > p.Nf ? e - p.Nf : 0);
+        if (ready > t_done) {
+            ag.t_begin = t_done;
+            HIP_TRY(timed(ctx, 2, s, [&] { return launch_aggregate((int)C, ag, ready - t_done, s); }));
+            if (ctx->prof) ctx->contributions += (double)(ready - t_done) * io.W * io.H * C;
+            t_done = ready;
+        }
+    }
+    return VBM3D_OK;
+}
+
+int flow_absmax(vbm3d_ctx* ctx, const float* ff, const float* bf, unsigned W, unsigned H,
+                unsigned F, hipStream_t s, unsigned* out) {
+    *out = 0;
+    if (!ff && !bf) return VBM3D_OK;
+    HIP_TRY(ensure(ctx->scal, 4));
+    HIP_TRY(hipMemsetAsync(ctx->scal.p, 0, 4, s));
+    const size_t n = (size_t)(F > 1 ? F - 1 : 0) * W * H * 2;
+    if (ff) HIP_TRY(launch_flow_absmax(ff, n, (unsigned*)ctx->scal.p, s));
+    if (bf) HIP_TRY(launch_flow_absmax(bf, n, (unsigned*)ctx->scal.p, s));
+    HIP_TRY(hipMemcpyAsync(out, ctx->scal.p, 4, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return VBM3D_OK;
+}
+
+// run_vbm3d on device buffers (src/vbm3d.cpp:87-376).
+int run_device(vbm3d_ctx* ctx, float sigma, float* noisy, const float* ff, const float* bf,
+               unsigned W, unsigned H, unsigned F, unsigned C, const vbm3d_params* p1,
+               const vbm3d_params* p2, int cs, int nb, float* basic, float* den, hipStream_t s) {
+    if (!ctx || !noisy || !basic || !den || !p1 || !p2) return fail(VBM3D_ERR_ARG, "null argument");
+    if (C != 1 && C != 3) return fail(VBM3D_ERR_ARG, "only 1 or 3 channels are supported");
+    if (p1->k > 0) { int rc = check(*p1, W, H, F, C, 1); if (rc) return rc; }
+    if (p2->k > 0) { int rc = check(*p2, W, H, F, C, 2); if (rc) return rc; }
+    unsigned maxflow = 0;
+    int rc = flow_absmax(ctx, ff, bf, W, H, F, s, &maxflow);
+    if (rc) return rc;
+    if (cs && C == 3)
+        HIP_TRY(timed(ctx, 3, s, [&] { return launch_color(p1->k > 0 ? noisy : basic, W, H, F, true, s); }));
+    StepIO io{};
+    io.sigma = sigma; io.noisy = noisy; io.ff = ff; io.bf = bf;
+    io.W = W; io.H = H; io.F = F; io.C = C; io.cs = cs != 0; io.nb = nb; io.maxflow = maxflow;
+    if (p1->k > 0) {
+        io.step = 1; io.p = p1; io.basic = nullptr;
+        io.out = basic; io.fallback = noisy;  // den == 0 -> noisy (:169, :277)
+        rc = run_step(ctx, io, s);
+        if (rc) return rc;
+    }
+    if (p2->k > 0) {
+        io.step = 2; io.p = p2; io.basic = basic;
+        io.out = den; io.fallback = nb == 1 ? basic : noisy;  // (:200, :343)
+        rc = run_step(ctx, io, s);
+        if (rc) return rc;
+    } else {
+        HIP_TRY(hipMemcpyAsync(den, basic, (size_t)W * H * F * C * 4, hipMemcpyDeviceToDevice, s));
+    }
+    if (cs && C == 3) {
+        HIP_TRY(timed(ctx, 3, s, [&] { return launch_color(den, W, H, F, false, s); }));
+        HIP_TRY(timed(ctx, 3, s, [&] { return launch_color(noisy, W, H, F, false, s); }));
+        HIP_TRY(timed(ctx, 3, s, [&] { return launch_color(basic, W, H, F, false, s); }));
+    }
+    HIP_TRY(hipStreamSynchronize(s));
+    return VBM3D_OK;
+}
+
+hipError_t upload(DevBuf& b, const float* h, size_t n, hipStream_t s) {
+    hipError_t e = ensure(b, n * 4);
+    if (e != hipSuccess || !h) return e;
+    return hipMemcpyAsync(b.p, h, n * 4, hipMemcpyHostToDevice, s);
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* vbm3d_last_error(void) { return g_err.c_str(); }
+
+vbm3d_ctx* vbm3d_create(int device) {
+    if (hipSetDevice(device) != hipSuccess) {
+        g_err = "hipSetDevice failed";
+        return nullptr;
+    }
+    vbm3d_ctx* c = new vbm3d_ctx();
+    c->device = device;
+    if (hipStreamCreate(&c->stream) != hipSuccess) {
+        g_err = "hipStreamCreate failed";
+        delete c;
+        return nullptr;
+    }
+    if (const char* e = std::getenv("VBM3D_EST_BUDGET_MB")) c->est_budget = (size_t)std::atol(e) << 20;
+    return c;
+}
+
+void vbm3d_destroy(vbm3d_ctx* c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    for (DevBuf* b : {&c->noisy, &c->basic, &c->den, &c->ff, &c->bf, &c->num_raw, &c->den_raw,
+                      &c->idx, &c->nsx, &c->nsim, &c->dist, &c->est, &c->wgt, &c->tabs,
+                      &c->refxy, &c->meta, &c->gpos, &c->rlist, &c->scal})
+        if (b->p) (void)hipFree(b->p);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+void vbm3d_default_params_1(vbm3d_params* p, int k, int kt, int Nf, int Ns, int Npr, int Nb,
+                            int pstep, int N, int d, float tau, float lambda3D, unsigned T_2D,
+                            unsigned T_3D, float sigma) {
+    // initializeParameters_1 (src/main.cpp:35-125)
+    p->k = k < 0 ? 8 : k;
+    p->kt = kt < 0 ? 1 : kt;
+    p->Nf = Nf < 0 ? 4 : Nf;
+    p->Ns = Ns < 0 ? 7 : Ns;
+    p->Npr = Npr < 0 ? 5 : Npr;
+    p->Nb = Nb < 0 ? 2 : Nb;
+    p->d = (float)(d < 0 ? (7. * 7. * 255.) / (p->k * p->k * p->kt) : (d * d * 255.) / (p->k * p->k * p->kt));
+    p->p = pstep < 0 ? p->k / 4 * 3 : pstep;
+    p->N = N < 0 ? 8 : N;
+    p->tau = tau < 0 ? ((sigma > 30) ? 4500.f : 3000.f) : tau;
+    p->lambda3D = lambda3D < 0 ? 2.7f : lambda3D;
+    p->T_2D = T_2D == 0xFFFFFFFFu ? (p->k == 8 ? VBM3D_BIOR : VBM3D_DCT) : (p->k == 8 ? T_2D : VBM3D_DCT);
+    p->T_3D = T_3D == 0xFFFFFFFFu ? VBM3D_HAAR : T_3D;
+    p->mc = 0;
+    p->n = 16;
+}
+
+void vbm3d_default_params_2(vbm3d_params* p, int k, int kt, int Nf, int Ns, int Npr, int Nb,
+                            int pstep, int N, int d, float tau, unsigned T_2D, unsigned T_3D,
+                            float sigma) {
+    // initializeParameters_2 (src/main.cpp:127-207)
+    p->k = k < 0 ? ((sigma > 30) ? 8 : 7) : k;
+    p->kt = kt < 0 ? 1 : kt;
+    p->Nf = Nf < 0 ? 4 : Nf;
+    p->Ns = Ns < 0 ? 7 : Ns;
+    p->Npr = Npr < 0 ? 5 : Npr;
+    p->Nb = Nb < 0 ? 2 : Nb;
+    p->d = (float)(d < 0 ? (3. * 3. * 255.) / (p->k * p->k * p->kt) : (d * d * 255.) / (p->k * p->k * p->kt));
+    p->p = pstep < 0 ? p->k / 2 : pstep;
+    p->N = N < 0 ? 8 : N;
+    p->tau = tau < 0 ? ((sigma > 30) ? 3000.f : 1500.f) : tau;
+    p->lambda3D = 0.f;  // src/parameters.h:51
+    p->T_2D = T_2D == 0xFFFFFFFFu ? VBM3D_DCT : (p->k == 8 ? T_2D : VBM3D_DCT);
+    p->T_3D = T_3D == 0xFFFFFFFFu ? VBM3D_HAAR : T_3D;
+    p->mc = 0;
+    p->n = 16;
+}
+
+long vbm3d_grid(unsigned W, unsigned H, unsigned k, unsigned p, int n, int nb, unsigned* xs,
+                unsigned* ys, unsigned* crop) {
+    GridDef g;
+    if (!build_grid(g, W, H, k, p, n, nb)) return -1;
+    long cnt = 0;
+    for (size_t c = 0; c < g.rows.size(); c++)
+        for (unsigned y : g.rows[c])
+            for (unsigned x : g.cols[c]) {
+                if (xs) xs[cnt] = x;
+                if (ys) ys[cnt] = y;
+                if (crop) crop[cnt] = (unsigned)c;
+                cnt++;
+            }
+    return cnt;
+}
+
+int vbm3d_run_device(vbm3d_ctx* ctx, float sigma, float* d_noisy, const float* d_fflow,
+                     const float* d_bflow, unsigned W, unsigned H, unsigned F, unsigned C,
+                     const vbm3d_params* p1, const vbm3d_params* p2, int color_space,
+                     int ref_grid_nb, float* d_basic, float* d_denoised, void* stream) {
+    if (!ctx) return fail(VBM3D_ERR_ARG, "null context");
+    HIP_TRY(hipSetDevice(ctx->device));
+    hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
+    return run_device(ctx, sigma, d_noisy, d_fflow, d_bflow, W, H, F, C, p1, p2, color_space,
+                      ref_grid_nb, d_basic, d_denoised, s);
+}
+
+int vbm3d_run(vbm3d_ctx* ctx, float sigma, float* noisy, const float* fflow, const float* bflow,
+              unsigned W, unsigned H, unsigned F, unsigned C, const vbm3d_params* p1,
+              const vbm3d_params* p2, int color_space, int ref_grid_nb, float* basic,
+              float* denoised) {
+    if (!ctx || !noisy || !basic || !denoised || !p1 || !p2) return fail(VBM3D_ERR_ARG, "null argument");
+    HIP_TRY(hipSetDevice(ctx->device));
+    hipStream_t s = ctx->stream;
+    const size_t n = (size_t)W * H * F * C, nf = (size_t)(F > 1 ? F - 1 : 1) * W * H * 2;
+    HIP_TRY(upload(ctx->noisy, noisy, n, s));
+    HIP_TRY(upload(ctx->basic, p1->k > 0 ? nullptr : basic, n, s));
+    HIP_TRY(ensure(ctx->den, n * 4));
+    if (fflow) HIP_TRY(upload(ctx->ff, fflow, nf, s));
+    if (bflow) HIP_TRY(upload(ctx->bf, bflow, nf, s));
+    int rc = run_device(ctx, sigma, (float*)ctx->noisy.p, fflow ? (float*)ctx->ff.p : nullptr,
+                        bflow ? (float*)ctx->bf.p : nullptr, W, H, F, C, p1, p2, color_space,
+                        ref_grid_nb, (float*)ctx->basic.p, (float*)ctx->den.p, s);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpy(noisy, ctx->noisy.p, n * 4, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(basic, ctx->basic.p, n * 4, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(denoised, ctx->den.p, n * 4, hipMemcpyDeviceToHost));
+    return VBM3D_OK;
+}
+
+int vbm3d_step(vbm3d_ctx* ctx, int step, float sigma, const float* noisy, const float* basic,
+               const float* fflow, const float* bflow, unsigned W, unsigned H, unsigned F,
+               unsigned C, const vbm3d_params* prm, int color_space, int ref_grid_nb, float* num,
+               float* den) {
+    if (!ctx || !noisy || !prm || !num || !den || (step == 2 && !basic) || (step != 1 && step != 2))
+        return fail(VBM3D_ERR_ARG, "invalid argument");
+    HIP_TRY(hipSetDevice(ctx->device));
+    hipStream_t s = ctx->stream;
+    const size_t n = (size_t)W * H * F * C, nf = (size_t)(F > 1 ? F - 1 : 1) * W * H * 2;
+    int rc = check(*prm, W, H, F, C, step);
+    if (rc) return rc;
+    HIP_TRY(upload(ctx->noisy, noisy, n, s));
+    if (step == 2) HIP_TRY(upload(ctx->basic, basic, n, s));
+    if (fflow) HIP_TRY(upload(ctx->ff, fflow, nf, s));
+    if (bflow) HIP_TRY(upload(ctx->bf, bflow, nf, s));
+    HIP_TRY(ensure(ctx->num_raw, n * 4));
+    HIP_TRY(ensure(ctx->den_raw, n * 4));
+    const float* dff = fflow ? (const float*)ctx->ff.p : nullptr;
+    const float* dbf = bflow ? (const float*)ctx->bf.p : nullptr;
+    unsigned maxflow = 0;
+    rc = flow_absmax(ctx, dff, dbf, W, H, F, s, &maxflow);
+    if (rc) return rc;
+    StepIO io{};
+    io.step = step; io.sigma = sigma;
+    io.noisy = (const float*)ctx->noisy.p;
+    io.basic = step == 2 ? (const float*)ctx->basic.p : nullptr;
+    io.ff = dff; io.bf = dbf;
+    io.W = W; io.H = H; io.F = F; io.C = C; io.p = prm; io.cs = color_space != 0; io.nb = ref_grid_nb;
+    io.out = nullptr; io.fallback = nullptr;
+    io.num_raw = (float*)ctx->num_raw.p;
+    io.den_raw = (float*)ctx->den_raw.p;
+    io.maxflow = maxflow;
+    rc = run_step(ctx, io, s);
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(s));
+    HIP_TRY(hipMemcpy(num, ctx->num_raw.p, n * 4, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(den, ctx->den_raw.p, n * 4, hipMemcpyDeviceToHost));
+    return VBM3D_OK;
+}
+
+int vbm3d_block_match(vbm3d_ctx* ctx, const float* vid, unsigned W, unsigned H, unsigned F,
+                      unsigned C, const float* fflow, const float* bflow,
+                      const vbm3d_params* prm, long nrefs, const uint32_t* ref_idx,
+                      uint32_t* idx_out, float* dist_out, uint32_t* nsim_out, uint32_t* nsx_out) {
+    if (!ctx || !vid || !prm || !ref_idx || !idx_out || !dist_out || !nsim_out || !nsx_out || nrefs < 0)
+        return fail(VBM3D_ERR_ARG, "invalid argument");
+    int rc = check(*prm, W, H, F, C, 0);
+    if (rc) return rc;
+    const unsigned whc = W * H * C;
+    for (long r = 0; r < nrefs; r++) {
+        const unsigned t = ref_idx[r] / whc, rem = ref_idx[r] % whc;
+        if (t >= F || rem >= W * H || rem % W > W - prm->k || rem / W > H - prm->k)
+            return fail(VBM3D_ERR_ARG, "reference index outside the valid patch range");
+    }
+    if (nrefs == 0) return VBM3D_OK;
+    HIP_TRY(hipSetDevice(ctx->device));
+    hipStream_t s = ctx->stream;
+    const size_t n = (size_t)W * H * F * C, nf = (size_t)(F > 1 ? F - 1 : 1) * W * H * 2;
+    HIP_TRY(upload(ctx->noisy, vid, n, s));
+    if (fflow) HIP_TRY(upload(ctx->ff, fflow, nf, s));
+    if (bflow) HIP_TRY(upload(ctx->bf, bflow, nf, s));
+    HIP_TRY(ensure(ctx->rlist, nrefs * 4));
+    HIP_TRY(ensure(ctx->idx, nrefs * prm->N * 4));
+    HIP_TRY(ensure(ctx->dist, nrefs * prm->N * 4));
+    HIP_TRY(ensure(ctx->nsim, nrefs * 4));
+    HIP_TRY(ensure(ctx->nsx, nrefs * 4));
+    HIP_TRY(hipMemcpyAsync(ctx->rlist.p, ref_idx, nrefs * 4, hipMemcpyHostToDevice, s));
+    BMArgs bm{};
+    bm.vid = (const float*)ctx->noisy.p;
+    bm.ff = fflow ? (const float*)ctx->ff.p : nullptr;
+    bm.bf = bflow ? (const float*)ctx->bf.p : nullptr;
+    bm.W = W; bm.H = H; bm.F = F; bm.C = C; bm.wh = W * H; bm.whc = whc;
+    bm.ref_list = (const uint32_t*)ctx->rlist.p;
+    bm.r_begin = 0; bm.r_end = nrefs;
+    bm.nrf = 1; bm.ring = 1;
+    bm.Ns = prm->Ns; bm.Npr = prm->Npr; bm.Nb = prm->Nb; bm.Nf = prm->Nf; bm.N = prm->N; bm.kt = prm->kt;
+    bm.d = prm->d; bm.tau = prm->tau;
+    bm.idx_out = (uint32_t*)ctx->idx.p;
+    bm.dist_out = (float*)ctx->dist.p;
+    bm.nsim_out = (uint32_t*)ctx->nsim.p;
+    bm.nsx_out = (uint32_t*)ctx->nsx.p;
+    HIP_TRY(launch_bm((int)prm->k, bm, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    HIP_TRY(hipMemcpy(idx_out, ctx->idx.p, nrefs * prm->N * 4, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(dist_out, ctx->dist.p, nrefs * prm->N * 4, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(nsim_out, ctx->nsim.p, nrefs * 4, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(nsx_out, ctx->nsx.p, nrefs * 4, hipMemcpyDeviceToHost));
+    return VBM3D_OK;
+}
+
+int vbm3d_profile(vbm3d_ctx* ctx, int enable) {
+    if (!ctx) return fail(VBM3D_ERR_ARG, "null context");
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    for (ProfEvent& e : ctx->events) {
+        if (e.a) (void)hipEventDestroy(e.a);
+        if (e.b) (void)hipEventDestroy(e.b);
+    }
+    ctx->events.clear();
+    ctx->refs_filtered = ctx->contributions = 0;
+    ctx->prof = enable != 0;
+    if (ctx->prof) {
+        HIP_TRY(ensure(ctx->stats, 4 * sizeof(unsigned long long)));
+        HIP_TRY(hipMemset(ctx->stats.p, 0, 4 * sizeof(unsigned long long)));
+    }
+    return VBM3D_OK;
+}
+
+int vbm3d_profile_read(vbm3d_ctx* ctx, int kernel, double* ms, long* launches, double* work) {
+    if (!ctx || !ms || !launches || !work) return fail(VBM3D_ERR_ARG, "null argument");
+    HIP_TRY(hipSetDevice(ctx->device));
+    for (ProfEvent& e : ctx->events) HIP_TRY(hipEventSynchronize(e.b));
+    double tot = 0;
+    long n = 0;
+    for (ProfEvent& e : ctx->events)
+        if (e.kernel == kernel) {
+            float t = 0;
+            HIP_TRY(hipEventElapsedTime(&t, e.a, e.b));
+            tot += t;
+            n++;
+        }
+    unsigned long long st[4] = {0, 0, 0, 0};
+    if (ctx->stats.p) HIP_TRY(hipMemcpy(st, ctx->stats.p, sizeof st, hipMemcpyDeviceToHost));
+    *ms = tot;
+    *launches = n;
+    *work = kernel == 0 ? (double)st[0] : kernel == 1 ? (double)st[1]
+          : kernel == 2 ? ctx->contributions : kernel == 4 ? (double)st[2] : 0.0;
+    return VBM3D_OK;
+}
+
+void vbm3d_selftest_partial_sort(float* dist, uint32_t* idx, unsigned len, unsigned mid) {
+    std::vector<VbEnt> v(len);
+    for (unsigned i = 0; i < len; i++) v[i] = VbEnt{dist[i], idx[i]};
+    vb_partial_sort(v.data(), (int)len, (int)std::min(mid, len));
+    for (unsigned i = 0; i < len; i++) {
+        dist[i] = v[i].d;
+        idx[i] = v[i].i;
+    }
+}
+
+}  // extern "C"

@@ -1,0 +1,91 @@
+// kernels.h -- device-side argument blocks and launch helpers shared by
+// the HIP kernels (kernels.hip) and the host driver (capi.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace vb {
+
+constexpr int kMaxN = 16;       // largest group size the kernels hold in registers
+constexpr int kMaxWin = 8;      // largest Ns / Npr (window <= 64 candidates)
+constexpr int kMaxBest = 64;    // Nb * (2 Nf + 1) bound
+constexpr int kMaxSlots = 512;  // Nb * Npr^2 bound
+constexpr int kAggTile = 16;    // aggregation tile edge (pixels)
+
+// Transform tables of one step, uploaded once per step.
+struct Tabs {
+    float biorF[64];   // analysis rows [j][s]
+    float biorI[64];   // synthesis [s][j]
+    float dctF[64];    // REDFT10 matrix [v][q]
+    float dctI[64];    // REDFT01 matrix [q][v]
+    float nrm[64];     // coef_norm
+    float inv[64];     // coef_norm_inv
+    float kaiser[64];  // aggregation window
+    float dct_out;     // 1/(2k)
+};
+
+struct BMArgs {
+    const float* vid;  // matching video [F][C][H][W]
+    const float* ff;   // forward flow [F-1][2][H][W] or null
+    const float* bf;   // backward flow or null
+    unsigned W, H, F, C, wh, whc;
+    unsigned nrf;             // reference patches per frame (grid mode)
+    const uint32_t* ref_xy;   // grid mode: x | y << 16 per in-frame ref
+    const uint32_t* ref_list; // list mode: global patch indices (overrides grid)
+    long r_begin, r_end;      // global ref ids t*nrf + i (or list positions)
+    unsigned ring;            // slot = (t % ring) * nrf + i (grid mode)
+    int Ns, Npr, Nb, Nf, N, kt;
+    float d, tau;
+    uint32_t* idx_out;   // [slot][N]
+    float* dist_out;     // [slot][N] or null
+    uint32_t* nsim_out;  // [slot] or null
+    uint32_t* nsx_out;   // [slot]
+    unsigned long long* stats;  // null, or [0] += candidates evaluated, [1] += group members,
+                                // [2] += pixel comparisons (candidates * K*K*C)
+};
+
+struct FArgs {
+    const float* noisy;
+    const float* basic;  // step 2 only
+    unsigned W, H, F, C, wh, whc;
+    unsigned nrf, ring;
+    long r_begin, r_end;
+    const uint32_t* idx;
+    const uint32_t* nsx;
+    int N, T2D, T3D;
+    float lambda;
+    float sig[3];
+    const Tabs* tabs;
+    float* est;  // [slot][C][N][K*K]
+    float* wgt;  // [slot][C]
+};
+
+struct AArgs {
+    unsigned W, H, F, C, wh, whc;
+    int K, N, Nf, kt;
+    unsigned nrf, ring, Fr;
+    int ncrop;
+    const int* crop_meta;       // per crop: offset in frame list, nrows, ncols, rows_at, cols_at
+    const uint16_t* grid_pos;   // concatenated sorted row and column positions
+    const uint32_t* idx;
+    const uint32_t* nsx;
+    const float* est;
+    const float* wgt;
+    const Tabs* tabs;
+    int R;                      // largest member-to-reference displacement
+    unsigned t_begin;           // first output frame of this launch
+    const float* fallback;      // used where den == 0
+    float* out;                 // out = den == 0 ? fallback : num / den
+    float* num_out;             // optional raw sums
+    float* den_out;
+};
+
+// Launchers (kernels.hip).  Return hipSuccess or the launch error.
+hipError_t launch_bm(int K, const BMArgs& a, hipStream_t s);
+hipError_t launch_filter(int K, int step, const FArgs& a, hipStream_t s);
+hipError_t launch_aggregate(int C, const AArgs& a, unsigned frames, hipStream_t s);
+hipError_t launch_color(float* v, unsigned W, unsigned H, unsigned F, bool forward,
+                        hipStream_t s);
+hipError_t launch_flow_absmax(const float* fl, size_t n, unsigned* out, hipStream_t s);
+
+}  // namespace vb

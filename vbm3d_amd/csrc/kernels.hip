@@ -1,0 +1,823 @@
+// kernels.hip -- CDNA4 (gfx950) kernels of the VBM3D collaborative filter.
+//
+// Every kernel reproduces the reference's FP32 operation order so that the
+// GPU result is bit-identical to the sequential CPU restatement:
+//   * no FMA contraction (file-wide pragma + -ffp-contract=off);
+//   * float sums accumulated sequentially in the reference's index order;
+//   * IEEE-correct float division (hipcc's default);
+//   * the Haar butterflies multiply a float sum by a double constant and
+//     round once to float, as src/lib_transforms.cpp:468-469 does.
+// Work decomposition (one wavefront = 64 lanes):
+//   k_bm         one wavefront per reference patch; lanes evaluate the
+//                candidates of a search window in parallel, selection is a
+//                wave-wide arg-min with a tie check (exact heap-select
+//                fallback in one lane when distances tie);
+//   k_filter     one wavefront per reference patch; lane = coefficient
+//                position, the group dimension lives in registers;
+//   k_aggregate  one 16x16 output tile per workgroup; contributions are
+//                gathered in the reference's (crop, frame, row, column, n)
+//                order, so every pixel's sum is formed in the same order as
+//                the CPU and no atomics are used.
+#include "kernels.h"
+#include "select.h"
+
+#pragma clang fp contract(off)
+
+namespace vb {
+namespace {
+
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Monotone map float -> uint32 (for finite inputs); -0 and +0 map alike so
+// that equal floats give equal keys.
+__device__ __forceinline__ unsigned fkey(float f) {
+    const unsigned u = __float_as_uint(f + 0.0f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+__device__ __forceinline__ unsigned wave_min_u32(unsigned v) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v = min(v, (unsigned)__shfl_xor((int)v, o, 64));
+    return v;
+}
+
+__device__ __forceinline__ int wave_sum_i32(int v) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+__device__ __forceinline__ unsigned lanes_below(unsigned long long m, int lane) {
+    return (unsigned)__popcll(m & ((1ull << lane) - 1ull));
+}
+
+// The `mid` smallest of list (ld, li)[0..m), m <= 64, in std::partial_sort
+// order, written to (od, oi)[0..mid).  Fast path: repeated wave arg-min; a
+// value equal to another remaining value at ranks < mid means the heap's
+// tie handling decides, and lane 0 then replays the heap select exactly.
+__device__ void wave_select(const float* ld, const uint32_t* li, int m, int mid,
+                            float* od, uint32_t* oi, VbEnt* scr, int lane) {
+    if (mid > 0) {
+        unsigned key = lane < m ? fkey(ld[lane]) : 0xFFFFFFFFu;
+        bool tie = false;
+        for (int j = 0; j < mid; j++) {
+            const unsigned mk = wave_min_u32(key);
+            const unsigned long long bal = __ballot(key == mk);
+            if (__popcll(bal) != 1) { tie = true; break; }
+            const int pos = __ffsll((long long)bal) - 1;
+            if (lane == 0) { od[j] = ld[pos]; oi[j] = li[pos]; }
+            if (lane == pos) key = 0xFFFFFFFFu;
+        }
+        if (tie) {
+            wave_sync();
+            if (lane == 0) {
+                for (int q = 0; q < m; q++) { scr[q].d = ld[q]; scr[q].i = li[q]; }
+                vb_partial_sort(scr, m, mid);
+                for (int q = 0; q < mid; q++) { od[q] = scr[q].d; oi[q] = scr[q].i; }
+            }
+        }
+    }
+    wave_sync();
+}
+
+// Sum of squared differences over C channels of two KxK patches, in the
+// reference's (c, y, x) order (src/vbm3d.cpp:965-973).
+template <int K, int C>
+__device__ __forceinline__ float patch_ssd(const float* __restrict__ v, unsigned rb,
+                                           unsigned qb, unsigned W, unsigned wh) {
+    float acc = 0.f;
+#pragma unroll
+    for (int c = 0; c < C; ++c)
+#pragma unroll
+        for (int y = 0; y < K; ++y)
+#pragma unroll
+            for (int x = 0; x < K; ++x) {
+                const float dif = v[rb + c * wh + y * W + x] - v[qb + c * wh + y * W + x];
+                acc = acc + dif * dif;
+            }
+    return acc;
+}
+
+// ---------------------------------------------------------------------
+// Predictive block matching: computeSimilarPatches / localSearch /
+// patchDistance (src/vbm3d.cpp:930-1197).
+// ---------------------------------------------------------------------
+template <int K, int C>
+__global__ __launch_bounds__(256) void k_bm(BMArgs a) {
+    __shared__ float s_d[4][kMaxSlots];
+    __shared__ uint32_t s_i[4][kMaxSlots];
+    __shared__ uint8_t s_v[4][kMaxSlots];
+    __shared__ float c_d[4][64];
+    __shared__ uint32_t c_i[4][64];
+    __shared__ float b_d[4][kMaxBest];
+    __shared__ uint32_t b_i[4][kMaxBest];
+    __shared__ float f_d[4][64];
+    __shared__ uint32_t f_i[4][64];
+    __shared__ float o_d[4][64];
+    __shared__ uint32_t o_i[4][64];
+    __shared__ uint32_t pp[4][2][8];  // [0] = backward chain (pre), [1] = forward (post)
+    __shared__ VbEnt scr[4][64];
+
+    const int lane = threadIdx.x & 63;
+    const int w = threadIdx.x >> 6;
+    const long r = a.r_begin + (long)blockIdx.x * 4 + w;
+    if (r >= a.r_end) return;  // wave-uniform
+    const unsigned W = a.W, H = a.H, wh = a.wh, whc = a.whc;
+    unsigned ridx, slot;
+    if (a.ref_list) {
+        ridx = a.ref_list[r];
+        slot = (unsigned)(r - a.r_begin);
+    } else {
+        const unsigned t = (unsigned)(r / a.nrf), i = (unsigned)(r - (long)t * a.nrf);
+        const unsigned xy = a.ref_xy[i];
+        ridx = t * whc + (xy >> 16) * W + (xy & 0xFFFFu);
+        slot = (t % a.ring) * a.nrf + i;
+    }
+    ridx = __builtin_amdgcn_readfirstlane(ridx);
+    slot = __builtin_amdgcn_readfirstlane(slot);
+    const unsigned rt = ridx / whc, ry = (ridx % wh) / W, rx = ridx % W;
+    const float norm = (float)(unsigned)(K * K * a.kt * C);
+
+    auto dist_of = [&](unsigned qx, unsigned qy, unsigned qt) -> float {
+        float s = patch_ssd<K, C>(a.vid, ridx, qt * whc + qy * W + qx, W, wh);
+        s = s / norm;
+        return s - ((qx == rx && qy == ry) ? a.d : 0.f);
+    };
+
+    // -- search in the reference frame (src/vbm3d.cpp:1114-1119)
+    int nbest;
+    unsigned n_eval = 0;
+    {
+        const int h = (a.Ns - 1) / 2;
+        const int x0 = max(0, (int)rx - h), y0 = max(0, (int)ry - h);
+        const int x1 = min((int)W - K, (int)rx + h), y1 = min((int)H - K, (int)ry + h);
+        const int wx = x1 - x0 + 1, m = wx * (y1 - y0 + 1);
+        if (lane < m) {
+            const unsigned qx = x0 + lane % wx, qy = y0 + lane / wx;
+            c_d[w][lane] = dist_of(qx, qy, rt);
+            c_i[w][lane] = rt * whc + qy * W + qx;
+        }
+        wave_sync();
+        n_eval += m;
+        const int nb = min(a.Nb, m);
+        wave_select(c_d[w], c_i[w], m, nb, o_d[w], o_i[w], scr[w], lane);
+        if (lane < nb) {
+            b_d[w][lane] = o_d[w][lane];
+            b_i[w][lane] = o_i[w][lane];
+            pp[w][0][lane] = o_i[w][lane];
+            pp[w][1][lane] = o_i[w][lane];
+        }
+        nbest = nb;
+        wave_sync();
+    }
+
+    // -- predictive search, forward (dir 1) then backward (dir 0)
+    //    (src/vbm3d.cpp:1121-1167)
+    const int P2 = a.Npr * a.Npr, hs = (a.Npr - 1) / 2;
+    const int nslots = a.Nb * P2;
+    const int nfw = (int)(min(rt + (unsigned)a.Nf, a.F - (unsigned)a.kt) - rt);
+    const int nbw = (int)rt - max((int)rt - a.Nf, 0);
+    for (int pass = 0; pass < 2; pass++) {
+        const int dir = pass == 0 ? 1 : 0;
+        const int nst = dir ? nfw : nbw;
+        const float* fl = dir ? a.ff : a.bf;
+        for (int f = 0; f < nst; f++) {
+            // window of temp match wm: flow projection (x first, then y at
+            // the updated x) and clamped Npr x Npr box
+            auto win = [&](int wm, int& x0, int& y0, int& wx, int& wy, unsigned& tf) {
+                const unsigned mi = pp[w][dir][wm];
+                unsigned px = mi % W, py = (mi % wh) / W;
+                const unsigned pt = mi / whc;
+                const unsigned ft = dir ? pt : pt - 1;
+                const unsigned hk = K / 2;
+                const float fx = fl ? fl[ft * 2 * wh + (py + hk) * W + px + hk] : 0.f;
+                px = (unsigned)min(max((int)((float)px + roundf(fx)), 0), (int)(W - K));
+                const float fy = fl ? fl[ft * 2 * wh + wh + (py + hk) * W + px + hk] : 0.f;
+                py = (unsigned)min(max((int)((float)py + roundf(fy)), 0), (int)(H - K));
+                tf = dir ? pt + 1 : pt - 1;
+                x0 = max(0, (int)px - hs);
+                y0 = max(0, (int)py - hs);
+                wx = min((int)W - K, (int)px + hs) - x0 + 1;
+                wy = min((int)H - K, (int)py + hs) - y0 + 1;
+            };
+            for (int base = 0; base < nslots; base += 64) {
+                const int s = base + lane;
+                if (s < nslots) {
+                    const int wm = s / P2, l = s - wm * P2;
+                    int x0, y0, wx, wy;
+                    unsigned tf;
+                    win(wm, x0, y0, wx, wy, tf);
+                    bool valid = false;
+                    float dv = 0.f;
+                    unsigned iv = 0;
+                    if (l < wx * wy) {
+                        const int qx = x0 + l % wx, qy = y0 + l / wx;
+                        // already seen = inside an earlier window of this frame
+                        bool seen = false;
+                        for (int m2 = 0; m2 < wm; m2++) {
+                            int ax, ay, aw, ah;
+                            unsigned at;
+                            win(m2, ax, ay, aw, ah, at);
+                            seen |= qx >= ax && qx < ax + aw && qy >= ay && qy < ay + ah;
+                        }
+                        if (!seen) {
+                            valid = true;
+                            dv = dist_of(qx, qy, tf);
+                            iv = tf * whc + qy * W + qx;
+                        }
+                    }
+                    s_d[w][s] = dv;
+                    s_i[w][s] = iv;
+                    s_v[w][s] = valid;
+                }
+            }
+            wave_sync();
+            // top Nb of each window's list, appended in window order
+            int nfb = 0;
+            for (int wm = 0; wm < a.Nb; wm++) {
+                const bool v = lane < P2 && s_v[w][wm * P2 + lane];
+                const unsigned long long bal = __ballot(v);
+                if (v) {
+                    const unsigned pos = lanes_below(bal, lane);
+                    c_d[w][pos] = s_d[w][wm * P2 + lane];
+                    c_i[w][pos] = s_i[w][wm * P2 + lane];
+                }
+                const int m = __popcll(bal);
+                n_eval += m;
+                wave_sync();
+                const int nb = min(a.Nb, m);
+                wave_select(c_d[w], c_i[w], m, nb, o_d[w], o_i[w], scr[w], lane);
+                if (lane < nb) {
+                    f_d[w][nfb + lane] = o_d[w][lane];
+                    f_i[w][nfb + lane] = o_i[w][lane];
+                }
+                nfb += nb;
+                wave_sync();
+            }
+            const int nb = min(a.Nb, nfb);
+            wave_select(f_d[w], f_i[w], nfb, nb, o_d[w], o_i[w], scr[w], lane);
+            if (lane < nb) {
+                pp[w][dir][lane] = o_i[w][lane];
+                b_d[w][nbest + lane] = o_d[w][lane];
+                b_i[w][nbest + lane] = o_i[w][lane];
+            }
+            nbest += nb;
+            wave_sync();
+        }
+    }
+
+    // -- N best overall, tau cut, power-of-two group size
+    //    (src/vbm3d.cpp:1169-1184)
+    const int nsim = min(a.N, nbest);
+    wave_select(b_d[w], b_i[w], nbest, nsim, o_d[w], o_i[w], scr[w], lane);
+    int it = nsim - 1;
+    while (o_d[w][it] > a.tau && it > 0) it--;
+    unsigned nsx = 1;
+    while (nsx * 2 <= (unsigned)(it + 1)) nsx *= 2;
+    if (lane < a.N) {
+        a.idx_out[(size_t)slot * a.N + lane] = lane < nsim ? o_i[w][lane] : 0xFFFFFFFFu;
+        if (a.dist_out)
+            a.dist_out[(size_t)slot * a.N + lane] = lane < nsim ? o_d[w][lane] : __builtin_nanf("");
+    }
+    if (lane == 0) {
+        a.nsx_out[slot] = nsx;
+        if (a.nsim_out) a.nsim_out[slot] = nsim;
+        if (a.stats) {
+            atomicAdd(&a.stats[0], (unsigned long long)n_eval);
+            atomicAdd(&a.stats[1], (unsigned long long)nsx);
+            atomicAdd(&a.stats[2], (unsigned long long)n_eval * (K * K * C));
+        }
+    }
+}
+
+// ---------------------------------------------------------------------
+// Group filtering.
+// ---------------------------------------------------------------------
+__device__ __forceinline__ float haar_scale(float s) {
+    return (float)((double)s * 0.7071067811865475);  // SQRT2_INV, src/vbm3d.cpp:33
+}
+
+// haar_forward (src/lib_transforms.cpp:453-476) on registers.
+template <int NS>
+__device__ __forceinline__ void haar_fwd(float* v) {
+    float tmp[NS > 1 ? NS : 1];
+#pragma unroll
+    for (int k = NS / 2; k >= 1; k /= 2) {
+#pragma unroll
+        for (int i = 0; i < k; i++) {
+            tmp[i] = haar_scale(v[2 * i] + v[2 * i + 1]);
+            tmp[i + k] = haar_scale(v[2 * i] - v[2 * i + 1]);
+        }
+#pragma unroll
+        for (int i = 0; i < 2 * k; i++) v[i] = tmp[i];
+    }
+}
+
+// haar_inverse (src/lib_transforms.cpp:489-512) on registers.
+template <int NS>
+__device__ __forceinline__ void haar_inv(float* v) {
+    float tmp[NS > 1 ? NS : 1];
+#pragma unroll
+    for (int k = 1; k * 2 <= NS; k *= 2) {
+#pragma unroll
+        for (int i = 0; i < k; i++) {
+            tmp[2 * i] = haar_scale(v[i] + v[i + k]);
+            tmp[2 * i + 1] = haar_scale(v[i] - v[i + k]);
+        }
+#pragma unroll
+        for (int i = 0; i < 2 * k; i++) v[i] = tmp[i];
+    }
+}
+
+// hadamard_transform, unnormalised (src/lib_transforms.cpp:410-440).
+template <int NS>
+__device__ __forceinline__ void hadamard(float* v) {
+    if constexpr (NS == 2) {
+        const float a = v[0], b = v[1];
+        v[0] = a + b;
+        v[1] = a - b;
+    } else if constexpr (NS > 2) {
+        constexpr int h = NS / 2;
+        float tmp[h];
+#pragma unroll
+        for (int k = 0; k < h; k++) {
+            const float a = v[2 * k], b = v[2 * k + 1];
+            v[k] = a + b;
+            tmp[k] = a - b;
+        }
+#pragma unroll
+        for (int k = 0; k < h; k++) v[h + k] = tmp[k];
+        hadamard<h>(v);
+        hadamard<h>(v + h);
+    }
+}
+
+template <int NS>
+__device__ __forceinline__ void t3_fwd(float* v, bool had) {
+    if (had) hadamard<NS>(v);
+    else haar_fwd<NS>(v);
+}
+template <int NS>
+__device__ __forceinline__ void t3_inv(float* v, bool had) {
+    if (had) hadamard<NS>(v);
+    else haar_inv<NS>(v);
+}
+
+// Forward 2D transform of the KxK patch in xs; returns coefficient `lane`
+// (bior_2d_forward src/lib_transforms.cpp:125-181, or the DCT with
+// coef_norm src/vbm3d.cpp:1245-1302).
+template <int K>
+__device__ float fwd2d(const float* xs, float* t2, int lane, bool bior, const Tabs& T) {
+    constexpr int K2 = K * K;
+    float out = 0.f;
+    if (bior) {
+        const int i = lane >> 3, j = lane & 7;
+        float m = 0.f;
+#pragma unroll
+        for (int s = 0; s < 8; s++) m = m + xs[i * 8 + s] * T.biorF[j * 8 + s];
+        t2[lane] = m;
+        wave_sync();
+        m = 0.f;
+#pragma unroll
+        for (int s = 0; s < 8; s++) m = m + t2[s * 8 + i] * T.biorF[j * 8 + s];
+        out = m;
+    } else {
+        const int p = lane / K, v = lane % K;
+        if (lane < K2) {
+            float m = 0.f;
+#pragma unroll
+            for (int q = 0; q < K; q++) m = m + xs[p * K + q] * T.dctF[v * K + q];
+            t2[lane] = m;
+        }
+        wave_sync();
+        if (lane < K2) {
+            float m = 0.f;
+#pragma unroll
+            for (int pp = 0; pp < K; pp++) m = m + t2[pp * K + v] * T.dctF[p * K + pp];
+            out = m * T.nrm[lane];
+        }
+    }
+    wave_sync();
+    return out;
+}
+
+// Inverse 2D transform of the coefficient vector held one per lane;
+// returns the value of pixel *pix (bior_2d_inverse
+// src/lib_transforms.cpp:270-321, dct_2d_inv src/vbm3d.cpp:1678-1713).
+template <int K>
+__device__ float inv2d(float cv, float* xs, float* t2, int lane, bool bior, const Tabs& T,
+                       int* pix) {
+    constexpr int K2 = K * K;
+    float out = 0.f;
+    if (bior) {
+        xs[lane] = cv;
+        wave_sync();
+        const int i = lane >> 3, j = lane & 7;
+        float m = 0.f;
+#pragma unroll
+        for (int s = 0; s < 8; s++) m = m + xs[i + s * 8] * T.biorI[j * 8 + s];
+        t2[lane] = m;
+        wave_sync();
+        m = 0.f;
+#pragma unroll
+        for (int s = 0; s < 8; s++) m = m + t2[s * 8 + i] * T.biorI[j * 8 + s];
+        *pix = j * 8 + i;
+        out = m;
+    } else {
+        if (lane < K2) xs[lane] = cv * T.inv[lane];
+        wave_sync();
+        const int p = lane / K, q = lane % K;
+        if (lane < K2) {
+            float m = 0.f;
+#pragma unroll
+            for (int v = 0; v < K; v++) m = m + xs[p * K + v] * T.dctI[q * K + v];
+            t2[lane] = m;
+        }
+        wave_sync();
+        if (lane < K2) {
+            float m = 0.f;
+#pragma unroll
+            for (int u = 0; u < K; u++) m = m + t2[u * K + q] * T.dctI[p * K + u];
+            out = T.dct_out * m;
+        }
+        *pix = lane;
+    }
+    wave_sync();
+    return out;
+}
+
+// 1D transform along the group, shrinkage and inverse for one channel
+// (ht_filtering_haar/_hadamard src/vbm3d.cpp:1405-1521,
+// wiener_filtering_haar/_hadamard :1539-1666).  z (and e) hold lane kk's
+// coefficient for every group member, in LDS [n][64].  Returns the
+// aggregation weight of the channel (valid in every lane).
+template <int K, int NS, int STEP>
+__device__ float shrink(float* zg, float* eg, float* v2, int lane, bool had, float lambda,
+                        float sig) {
+    constexpr int K2 = K * K;
+    const bool act = lane < K2;
+    float z[NS], e[NS];
+#pragma unroll
+    for (int n = 0; n < NS; n++) {
+        z[n] = act ? zg[n * 64 + lane] : 0.f;
+        if (STEP == 2) e[n] = act ? eg[n * 64 + lane] : 0.f;
+    }
+    float weight;
+    if (STEP == 1) {
+        t3_fwd<NS>(z, had);
+        const float thr = had ? lambda * sig * sqrtf((float)NS) : lambda * sig;
+        int cnt = 0;
+#pragma unroll
+        for (int n = 0; n < NS; n++) {
+            const bool keep = act && ((n == 0 && lane == 0) || fabsf(z[n]) > thr);
+            if (keep) cnt++;
+            else z[n] = 0.f;
+        }
+        cnt = wave_sum_i32(cnt);
+        t3_inv<NS>(z, had);
+        if (had) {
+            const float coef = 1.0f / (float)NS;
+#pragma unroll
+            for (int n = 0; n < NS; n++) z[n] *= coef;
+        }
+        weight = 1.f / (sig * sig * (float)cnt);
+#pragma unroll
+        for (int n = 0; n < NS; n++)
+            if (act) zg[n * 64 + lane] = z[n];
+    } else {
+        t3_fwd<NS>(z, had);
+        t3_fwd<NS>(e, had);
+        const float s2 = sig * sig;
+        const float coef = 1.0f / (float)NS;
+#pragma unroll
+        for (int n = 0; n < NS; n++) {
+            float value = e[n] * e[n];
+            if (had) value = value * coef;
+            value /= (value + s2);
+            e[n] = had ? z[n] * value * coef : z[n] * value;
+            if (act) v2[lane * NS + n] = value * value;  // flat index n + kk*nSx
+        }
+        wave_sync();
+        // Wiener weight: sequential float sum over the flat index
+        // (src/vbm3d.cpp:1651), one lane
+        float acc = 0.f;
+        if (lane == 0)
+            for (int f = 0; f < K2 * NS; f++) acc = acc + v2[f];
+        acc = __shfl(acc, 0, 64);
+        t3_inv<NS>(e, had);
+        if (had) weight = acc > 0.0f ? 1.0f / (s2 * acc) : 1.0f;
+        else weight = 1.f / (s2 * acc);
+#pragma unroll
+        for (int n = 0; n < NS; n++)
+            if (act) eg[n * 64 + lane] = e[n];
+        wave_sync();
+    }
+    return weight;
+}
+
+template <int K, int C, int STEP>
+__global__ __launch_bounds__(256) void k_filter(FArgs a) {
+    constexpr int K2 = K * K;
+    __shared__ float xs[4][64], t2[4][64];
+    __shared__ float zg[4][kMaxN * 64];
+    __shared__ float eg[4][STEP == 2 ? kMaxN * 64 : 1];
+    __shared__ float v2[4][STEP == 2 ? kMaxN * 64 : 1];
+    const int lane = threadIdx.x & 63;
+    const int w = threadIdx.x >> 6;
+    const long r = a.r_begin + (long)blockIdx.x * 4 + w;
+    if (r >= a.r_end) return;
+    const unsigned t = (unsigned)(r / a.nrf), i = (unsigned)(r - (long)t * a.nrf);
+    const unsigned slot = __builtin_amdgcn_readfirstlane((t % a.ring) * a.nrf + i);
+    const unsigned W = a.W, wh = a.wh, whc = a.whc;
+    const int nsx = (int)__builtin_amdgcn_readfirstlane(a.nsx[slot]);
+    const Tabs& T = *a.tabs;
+    const bool bior = a.T2D == 1;
+    const bool had = a.T3D == 2;
+    const int pr = lane / K, pc = lane % K;
+    const uint32_t* mem = a.idx + (size_t)slot * a.N;
+
+    for (int c = 0; c < C; c++) {
+        // 2D transforms of every group member (table_2D[c][n][kk])
+        for (int n = 0; n < nsx; n++) {
+            const unsigned mi = mem[n];
+            const unsigned px = mi % W, py = (mi % wh) / W, pt = mi / whc;
+            const unsigned base = pt * whc + c * wh + (py + pr) * W + px + pc;
+            if (lane < K2) xs[w][lane] = a.noisy[base];
+            wave_sync();
+            const float zc = fwd2d<K>(xs[w], t2[w], lane, bior, T);
+            if (lane < K2) zg[w][n * 64 + lane] = zc;
+            if (STEP == 2) {
+                if (lane < K2) xs[w][lane] = a.basic[base];
+                wave_sync();
+                const float ec = fwd2d<K>(xs[w], t2[w], lane, bior, T);
+                if (lane < K2) eg[w][n * 64 + lane] = ec;
+            }
+        }
+        wave_sync();
+        float* est_grp = STEP == 2 ? eg[w] : zg[w];
+        float weight;
+        switch (nsx) {
+            case 1: weight = shrink<K, 1, STEP>(zg[w], eg[w], v2[w], lane, had, a.lambda, a.sig[c]); break;
+            case 2: weight = shrink<K, 2, STEP>(zg[w], eg[w], v2[w], lane, had, a.lambda, a.sig[c]); break;
+            case 4: weight = shrink<K, 4, STEP>(zg[w], eg[w], v2[w], lane, had, a.lambda, a.sig[c]); break;
+            case 8: weight = shrink<K, 8, STEP>(zg[w], eg[w], v2[w], lane, had, a.lambda, a.sig[c]); break;
+            default: weight = shrink<K, 16, STEP>(zg[w], eg[w], v2[w], lane, had, a.lambda, a.sig[c]); break;
+        }
+        wave_sync();
+        // inverse 2D per member and store [slot][c][n][pixel]
+        float* dst = a.est + (((size_t)slot * C + c) * a.N) * K2;
+        for (int n = 0; n < nsx; n++) {
+            const float cv = lane < K2 ? est_grp[n * 64 + lane] : 0.f;
+            int pix = 0;
+            const float val = inv2d<K>(cv, xs[w], t2[w], lane, bior, T, &pix);
+            if (lane < K2) dst[(size_t)n * K2 + pix] = val;
+        }
+        if (lane == 0) a.wgt[(size_t)slot * C + c] = weight;
+        wave_sync();
+    }
+}
+
+// ---------------------------------------------------------------------
+// Aggregation (src/vbm3d.cpp:612-650, 860-924) with the division and the
+// den == 0 fallback (:168-170, :198-202, :267-280, :333-346).
+// ---------------------------------------------------------------------
+__device__ __forceinline__ int lower_bound_u16(const uint16_t* a, int n, int v) {
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if ((int)a[mid] < v) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+template <int C>
+__global__ __launch_bounds__(256) void k_aggregate(AArgs a) {
+    constexpr int TS = kAggTile;
+    __shared__ uint32_t L_sn[256 * kMaxN];
+    __shared__ uint32_t L_pos[256 * kMaxN];
+    __shared__ int s_w[4];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wv = tid >> 6;
+    const int x0 = blockIdx.x * TS, y0 = blockIdx.y * TS;
+    const unsigned to = a.t_begin + blockIdx.z;
+    const int px = x0 + (tid % TS), py = y0 + (tid / TS);
+    const bool inside = px < (int)a.W && py < (int)a.H;
+    const int K = a.K, K2 = K * K, N = a.N;
+    const unsigned W = a.W, wh = a.wh, whc = a.whc;
+    float tn[C], td[C];
+#pragma unroll
+    for (int c = 0; c < C; c++) tn[c] = td[c] = 0.f;
+    const int trlo = max(0, (int)to - a.Nf), trhi = min((int)a.Fr - 1, (int)to + a.Nf);
+    const int rlo = y0 - a.R - (K - 1), rhi = y0 + TS - 1 + a.R;
+    const int clo = x0 - a.R - (K - 1), chi = x0 + TS - 1 + a.R;
+
+    for (int ci = 0; ci < a.ncrop; ci++) {
+        const int* meta = a.crop_meta + ci * 5;
+        const int off = meta[0], nrows = meta[1], ncols = meta[2];
+        const uint16_t* rows = a.grid_pos + meta[3];
+        const uint16_t* cols = a.grid_pos + meta[4];
+        const int ia = lower_bound_u16(rows, nrows, rlo), ib = lower_bound_u16(rows, nrows, rhi + 1);
+        const int ja = lower_bound_u16(cols, ncols, clo), jb = lower_bound_u16(cols, ncols, chi + 1);
+        const int nr = ib - ia, nc = jb - ja, ntr = trhi - trlo + 1;
+        const long total = (nr > 0 && nc > 0 && ntr > 0) ? (long)ntr * nr * nc : 0;
+        float pn[C], pd[C];
+#pragma unroll
+        for (int c = 0; c < C; c++) pn[c] = pd[c] = 0.f;
+
+        for (long cb = 0; cb < total; cb += 256) {
+            const long e = cb + tid;
+            uint32_t hits = 0, slot = 0;
+            if (e < total) {
+                const int jj = (int)(e % nc);
+                const long rest = e / nc;
+                const int ii = (int)(rest % nr);
+                const int tr = trlo + (int)(rest / nr);
+                slot = ((unsigned)tr % a.ring) * a.nrf + off + (ia + ii) * ncols + (ja + jj);
+                const int ns = (int)a.nsx[slot];
+                const uint32_t* mem = a.idx + (size_t)slot * N;
+                for (int n = 0; n < ns; n++) {
+                    const unsigned mi = mem[n];
+                    if (mi / whc != to) continue;
+                    const int qy = (int)((mi % wh) / W), qx = (int)(mi % W);
+                    if (qx <= x0 + TS - 1 && qx + K - 1 >= x0 && qy <= y0 + TS - 1 && qy + K - 1 >= y0)
+                        hits |= 1u << n;
+                }
+            }
+            // ordered compaction: exclusive block scan of the hit counts
+            const int cnt = __popc(hits);
+            int x = cnt;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const int y = __shfl_up(x, o, 64);
+                if (lane >= o) x += y;
+            }
+            if (lane == 63) s_w[wv] = x;
+            __syncthreads();
+            int base = 0;
+            for (int q = 0; q < wv; q++) base += s_w[q];
+            const int tot = s_w[0] + s_w[1] + s_w[2] + s_w[3];
+            int o = base + x - cnt;
+            for (int n = 0; n < N; n++)
+                if ((hits >> n) & 1u) {
+                    const unsigned mi = a.idx[(size_t)slot * N + n];
+                    L_sn[o] = slot * kMaxN + n;
+                    L_pos[o] = (mi % W) | (((mi % wh) / W) << 16);
+                    o++;
+                }
+            __syncthreads();
+            if (inside) {
+                for (int j = 0; j < tot; j++) {
+                    const uint32_t pos = L_pos[j];
+                    const int dx = px - (int)(pos & 0xFFFFu), dy = py - (int)(pos >> 16);
+                    if ((unsigned)dx < (unsigned)K && (unsigned)dy < (unsigned)K) {
+                        const uint32_t sn = L_sn[j];
+                        const uint32_t s = sn / kMaxN, n = sn % kMaxN;
+                        const int kk = dy * K + dx;
+                        const float kai = a.tabs->kaiser[kk];
+#pragma unroll
+                        for (int c = 0; c < C; c++) {
+                            const float kw = kai * a.wgt[(size_t)s * C + c];
+                            pn[c] = pn[c] + kw * a.est[(((size_t)s * C + c) * N + n) * K2 + kk];
+                            pd[c] = pd[c] + kw;
+                        }
+                    }
+                }
+            }
+            __syncthreads();
+        }
+#pragma unroll
+        for (int c = 0; c < C; c++) {
+            if (a.ncrop == 1) {
+                tn[c] = pn[c];
+                td[c] = pd[c];
+            } else {
+                tn[c] = tn[c] + pn[c];
+                td[c] = td[c] + pd[c];
+            }
+        }
+    }
+    if (inside) {
+#pragma unroll
+        for (int c = 0; c < C; c++) {
+            const size_t o = (size_t)to * whc + (size_t)c * wh + (size_t)py * W + px;
+            if (a.num_out) {
+                a.num_out[o] = tn[c];
+                a.den_out[o] = td[c];
+            }
+            if (a.out) a.out[o] = td[c] == 0.f ? a.fallback[o] : tn[c] / td[c];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------
+// Colour transform, 3 channels (src/Utilities/LibVideoT.hpp:777-887).
+// ---------------------------------------------------------------------
+__global__ void k_color(float* v, unsigned wh, unsigned F, int forward) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)wh * F) return;
+    const size_t f = i / wh, k = i - f * wh;
+    float* r = v + f * 3 * wh;
+    const float R = r[k], G = r[wh + k], B = r[2 * wh + k];
+    float o0, o1, o2;
+    if (forward) {
+        o0 = 0.299f * R + 0.587f * G + 0.114f * B;
+        o1 = -0.14713f * R - 0.28886f * G + 0.436f * B;
+        o2 = 0.615f * R - 0.51498f * G - 0.10001f * B;
+    } else {
+        o0 = R + 1.13983f * B;
+        o1 = R - 0.39465f * G - 0.5806f * B;
+        o2 = R + 2.03211f * G;
+    }
+    r[k] = o0;
+    r[wh + k] = o1;
+    r[2 * wh + k] = o2;
+}
+
+__global__ void k_flow_absmax(const float* fl, size_t n, unsigned* out) {
+    unsigned m = 0;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        m = max(m, (unsigned)fabsf(roundf(fl[i])));
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) m = max(m, (unsigned)__shfl_xor((int)m, o, 64));
+    if ((threadIdx.x & 63) == 0) atomicMax(out, m);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------
+// Launchers
+// ---------------------------------------------------------------------
+template <int K, int C>
+static hipError_t bm_k(const BMArgs& a, hipStream_t s) {
+    const long n = a.r_end - a.r_begin;
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL((k_bm<K, C>), dim3((unsigned)((n + 3) / 4)), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_bm(int K, const BMArgs& a, hipStream_t s) {
+#define VB_BM(KK)                                           \
+    case KK:                                                \
+        return a.C == 1 ? bm_k<KK, 1>(a, s) : bm_k<KK, 3>(a, s);
+    switch (K) {
+        VB_BM(4) VB_BM(5) VB_BM(6) VB_BM(7) VB_BM(8)
+        default: return hipErrorInvalidValue;
+    }
+#undef VB_BM
+}
+
+template <int K, int C, int STEP>
+static hipError_t filter_k(const FArgs& a, hipStream_t s) {
+    const long n = a.r_end - a.r_begin;
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL((k_filter<K, C, STEP>), dim3((unsigned)((n + 3) / 4)), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+template <int K>
+static hipError_t filter_kc(int step, const FArgs& a, hipStream_t s) {
+    if (a.C == 1) return step == 1 ? filter_k<K, 1, 1>(a, s) : filter_k<K, 1, 2>(a, s);
+    return step == 1 ? filter_k<K, 3, 1>(a, s) : filter_k<K, 3, 2>(a, s);
+}
+
+hipError_t launch_filter(int K, int step, const FArgs& a, hipStream_t s) {
+    switch (K) {
+        case 4: return filter_kc<4>(step, a, s);
+        case 5: return filter_kc<5>(step, a, s);
+        case 6: return filter_kc<6>(step, a, s);
+        case 7: return filter_kc<7>(step, a, s);
+        case 8: return filter_kc<8>(step, a, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_aggregate(int C, const AArgs& a, unsigned frames, hipStream_t s) {
+    if (frames == 0) return hipSuccess;
+    dim3 grid((a.W + kAggTile - 1) / kAggTile, (a.H + kAggTile - 1) / kAggTile, frames);
+    if (C == 1) hipLaunchKernelGGL(k_aggregate<1>, grid, dim3(256), 0, s, a);
+    else hipLaunchKernelGGL(k_aggregate<3>, grid, dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_color(float* v, unsigned W, unsigned H, unsigned F, bool forward,
+                        hipStream_t s) {
+    const size_t n = (size_t)W * H * F;
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_color, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, v,
+                       W * H, F, forward ? 1 : 0);
+    return hipGetLastError();
+}
+
+hipError_t launch_flow_absmax(const float* fl, size_t n, unsigned* out, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_flow_absmax, dim3(1024), dim3(256), 0, s, fl, n, out);
+    return hipGetLastError();
+}
+
+}  // namespace vb
