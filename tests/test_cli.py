@@ -1,0 +1,171 @@
+"""VBM3Ddenoising command line: option parsing and error exits (CPU) and an
+end-to-end run on files (GPU) checked against the oracle."""
+import os
+import re
+import struct
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import ROOT
+
+CLI = os.path.join(ROOT, "vbm3d_amd", "bin", "VBM3Ddenoising")
+
+
+def run_cli(args, cwd):
+    return subprocess.run([CLI, *args], cwd=cwd, capture_output=True, text=True, timeout=300)
+
+
+def printed_params(out):
+    blocks = {}
+    cur = None
+    for line in out.splitlines():
+        m = re.match(r"Parameters (.*) step:", line)
+        if m:
+            cur = blocks.setdefault(m.group(1), {})
+            continue
+        m = re.match(r"\s+(.*?)\s+= (.*)", line)
+        if cur is not None and m:
+            cur[m.group(1)] = m.group(2).strip()
+    return blocks
+
+
+def test_cli_no_input(tmp_path):
+    r = run_cli([], tmp_path)
+    assert r.returncode == 1
+    assert "no input images" in r.stderr
+
+
+@pytest.mark.parametrize("opt,val,msg", [("-T2dh", "3", "unknown T_2d_hard"), ("-T2dw", "2", "unknown T_2d_wien"),
+                                         ("-T3dh", "4", "unknown T_3d_hard"), ("-T3dw", "1", "unknown T_3d_wien")])
+def test_cli_bad_transform_codes(tmp_path, opt, val, msg):
+    # the accepted codes are DCT=0 BIOR=1 HADAMARD=2 HAAR=3 (src/main.cpp:27-30, 305-323)
+    r = run_cli(["-i", "x_%03d.tif", opt, val], tmp_path)
+    assert r.returncode == 1
+    assert msg in r.stderr
+
+
+def test_cli_parameter_defaults_and_int_parsing(tmp_path):
+    # parameters are printed before any file or GPU work; the missing input
+    # then aborts like the reference's uncaught runtime_error
+    r = run_cli(["-i", "missing_%03d.tif", "-sigma", "80/2", "-tauHard", "2500.7", "-lambda3d", "2.7",
+                 "-kWien", "8", "-T2dw", "1", "-NbHard", "3"], tmp_path)
+    assert r.returncode != 0
+    p = printed_params(r.stdout)
+    hard, wien = p["hard thresholding"], p["Wiener filtering"]
+    assert hard["threshold lambda3D"] == "2.000000"        # atoi("2.7")
+    assert hard["pred. search tau"] == "2500.000000"       # atoi("2500.7")
+    assert hard["pred. search Nb"] == "3"
+    assert hard["patch size k x k x kt"] == "8x8x1"
+    assert hard["processing step p"] == "6"
+    assert hard["transform T_2D"] == "1"                  # BIOR at k = 8
+    assert wien["patch size k x k x kt"] == "8x8x1"
+    assert wien["transform T_2D"] == "1"
+    assert wien["pred. search tau"] == "3000.000000"      # sigma = 80/2 = 40 > 30
+    assert wien["pred. search d"] == "35.859375"
+    assert wien["threshold lambda3D"] == "0.000000"
+
+
+def test_cli_defaults_sigma20(tmp_path):
+    r = run_cli(["-i", "missing_%03d.tif", "-sigma", "20"], tmp_path)
+    p = printed_params(r.stdout)
+    w = p["Wiener filtering"]
+    assert w["patch size k x k x kt"] == "7x7x1" and w["processing step p"] == "3"
+    assert w["transform T_2D"] == "0" and w["pred. search tau"] == "1500.000000"
+    h = p["hard thresholding"]
+    assert h["pred. search d"] == "195.234375" and h["pred. search tau"] == "3000.000000"
+
+
+def test_cli_skipped_step_and_bool(tmp_path):
+    r = run_cli(["-i", "missing_%03d.tif", "-kWien", "0", "-verbose", "OFF"], tmp_path)
+    assert "Parameters" not in r.stdout  # -verbose OFF silences the listing
+    r = run_cli(["-i", "missing_%03d.tif", "-kWien", "0"], tmp_path)
+    assert "Wiener filtering step skipped" in r.stdout
+
+
+def write_pfm(path, img):
+    """img: [C][H][W] float32 with C in (1, 3)."""
+    c, h, w = img.shape
+    il = np.ascontiguousarray(np.transpose(img, (1, 2, 0)))[::-1]  # bottom-to-top rows
+    with open(path, "wb") as f:
+        f.write(("PF" if c == 3 else "Pf").encode() + b"\n%d %d\n-1\n" % (w, h))
+        f.write(il.astype("<f4").tobytes())
+
+
+def read_pfm(path):
+    with open(path, "rb") as f:
+        tag = f.readline().strip()
+        w, h = map(int, f.readline().split())
+        scale = float(f.readline())
+        c = 3 if tag == b"PF" else 1
+        data = np.frombuffer(f.read(), dtype="<f4" if scale < 0 else ">f4").reshape(h, w, c)[::-1]
+    return np.ascontiguousarray(np.transpose(data, (2, 0, 1)))
+
+
+def read_tiff_float(path):
+    """Reader for the little-endian uncompressed chunky TIFFs the CLI writes."""
+    b = open(path, "rb").read()
+    assert b[:4] == b"II*\x00"
+    ifd = struct.unpack_from("<I", b, 4)[0]
+    n = struct.unpack_from("<H", b, ifd)[0]
+    tags = {}
+    for e in range(n):
+        tag, typ, cnt, val = struct.unpack_from("<HHII", b, ifd + 2 + 12 * e)
+        if typ == 3 and cnt == 1:
+            val &= 0xFFFF
+        tags[tag] = (typ, cnt, val)
+    w, h, spp = tags[256][2], tags[257][2], tags[277][2]
+    off = tags[273][2]
+    fmt_entry = tags.get(339, (3, 1, 1))
+    fmt = fmt_entry[2] if spp == 1 else struct.unpack_from("<H", b, fmt_entry[2])[0]
+    dt = "<f4" if fmt == 3 else "u1"
+    a = np.frombuffer(b, dtype=dt, count=w * h * spp, offset=off).astype(np.float32).reshape(h, w, spp)
+    return np.ascontiguousarray(np.transpose(a, (2, 0, 1)))
+
+
+@pytest.mark.gpu
+def test_cli_end_to_end_matches_oracle(tmp_path, oracle):
+    """Files in, files out: the CLI's basic and denoised frames equal the
+    oracle's run_vbm3d on the same noisy input (-add false)."""
+    import vbm3d_amd as V
+    from vbm3d_amd import synth
+    _, noisy = synth.noisy_clip(64, 48, 6, 3, 20.0, 21)
+    for t in range(noisy.shape[0]):
+        write_pfm(str(tmp_path / f"in_{t:03d}.pfm"), noisy[t])
+    r = run_cli(["-i", "in_%03d.pfm", "-f", "0", "-l", "5", "-sigma", "20", "-add", "false",
+                 "-bsic", "b_%03d.pfm", "-deno", "d_%03d.tiff"], tmp_path)
+    assert r.returncode == 0, r.stderr
+    basic = np.stack([read_pfm(str(tmp_path / f"b_{t:03d}.pfm")) for t in range(6)])
+    final = np.stack([read_tiff_float(str(tmp_path / f"d_{t:03d}.tiff")) for t in range(6)])
+    ob, of, _ = oracle.run_vbm3d(20.0, noisy, V.initialize_parameters_1(20.0), V.initialize_parameters_2(20.0))
+    np.testing.assert_array_equal(basic.view(np.uint32), ob.view(np.uint32))
+    np.testing.assert_array_equal(final.view(np.uint32), of.view(np.uint32))
+
+
+@pytest.mark.gpu
+def test_cli_resume_from_basic_and_measure(tmp_path, oracle):
+    """-kHard 0 -b reloads a basic estimate and skips step 1
+    (src/main.cpp:350-351); -add true writes the measure file."""
+    import vbm3d_amd as V
+    from vbm3d_amd import synth
+    clean, noisy = synth.noisy_clip(48, 40, 5, 1, 20.0, 4)
+    for t in range(5):
+        write_pfm(str(tmp_path / f"c_{t:03d}.pfm"), clean[t])
+        write_pfm(str(tmp_path / f"n_{t:03d}.pfm"), noisy[t])
+    r = run_cli(["-i", "n_%03d.pfm", "-l", "4", "-sigma", "20", "-add", "false", "-bsic", "b_%03d.pfm",
+                 "-deno", "d_%03d.pfm"], tmp_path)
+    assert r.returncode == 0, r.stderr
+    r = run_cli(["-i", "n_%03d.pfm", "-l", "4", "-sigma", "20", "-add", "false", "-kHard", "0",
+                 "-b", "b_%03d.pfm", "-bsic", "", "-deno", "r_%03d.pfm"], tmp_path)
+    assert r.returncode == 0, r.stderr
+    d1 = np.stack([read_pfm(str(tmp_path / f"d_{t:03d}.pfm")) for t in range(5)])
+    d2 = np.stack([read_pfm(str(tmp_path / f"r_{t:03d}.pfm")) for t in range(5)])
+    np.testing.assert_array_equal(d1.view(np.uint32), d2.view(np.uint32))
+    r = run_cli(["-i", "c_%03d.pfm", "-l", "4", "-sigma", "20", "-meas", "m.txt", "-bsic", "",
+                 "-deno", "e_%03d.pfm"], tmp_path)
+    assert r.returncode == 0, r.stderr
+    txt = open(tmp_path / "m.txt").read()
+    vals = dict(re.findall(r"-(\w+) = ([0-9.]+)", txt))
+    assert float(vals["sigma"]) == 20.0
+    assert float(vals["PSNR"]) > float(vals["PSNR_basic"]) - 0.5 > 25.0

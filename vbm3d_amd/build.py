@@ -31,13 +31,34 @@ FLAGS = [
 ]
 
 
-def build(verbose: bool = False) -> str:
-    tmp = LIB + ".tmp"
-    cmd = [HIPCC, *FLAGS, "-o", tmp, *SOURCES]
+HOST = os.path.join(HERE, "host")
+HOST_LIB = os.path.join(HERE, "libvbm3d_host.so")
+CLI = os.path.join(HERE, "bin", "VBM3Ddenoising")
+CXX = os.environ.get("CXX", "g++")
+CXXFLAGS = ["-O2", "-std=c++17", "-Wall", "-Wextra", f"-I{os.path.join(ROOT, 'include')}"]
+
+
+def _run(cmd, verbose):
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.check_call(cmd)
+
+
+def build(verbose: bool = False) -> str:
+    """Compile libvbm3d_hip.so (HIP, gfx950), then the C++ host library
+    libvbm3d_host.so (run_vbm3d over the C ABI) and the VBM3Ddenoising
+    command line, both plain host C++ linked against libvbm3d_hip.so."""
+    tmp = LIB + ".tmp"
+    _run([HIPCC, *FLAGS, "-o", tmp, *SOURCES], verbose)
     os.replace(tmp, LIB)
+    host_src = [os.path.join(HOST, "vbm3d_host.cpp"), os.path.join(HOST, "image_io.cpp")]
+    _run([CXX, *CXXFLAGS, "-fPIC", "-shared", "-o", HOST_LIB + ".tmp", *host_src,
+          f"-L{HERE}", "-l:libvbm3d_hip.so", "-Wl,-rpath,$ORIGIN"], verbose)
+    os.replace(HOST_LIB + ".tmp", HOST_LIB)
+    os.makedirs(os.path.dirname(CLI), exist_ok=True)
+    _run([CXX, *CXXFLAGS, "-o", CLI + ".tmp", os.path.join(HOST, "main.cpp"),
+          f"-L{HERE}", "-l:libvbm3d_host.so", "-l:libvbm3d_hip.so", "-Wl,-rpath,$ORIGIN/.."], verbose)
+    os.replace(CLI + ".tmp", CLI)
     return LIB
 
 

@@ -1,0 +1,28 @@
+/*
+ * image_io.h -- frame and flow files for the VBM3Ddenoising command line.
+ *
+ * Formats, chosen by extension: .tif/.tiff (uncompressed, 8/16-bit
+ * integer or 32-bit float samples, chunky), .pfm, .pgm/.ppm (8/16-bit),
+ * and .flo (Middlebury optical flow).  Frames are returned planar
+ * [c][y][x] like the reference's loadImage (src/Utilities/LibImages.cpp:
+ * 47-110), including its rule that three equal channels collapse to one
+ * and a fourth (alpha) channel is dropped.
+ */
+#ifndef VBM3D_AMD_IMAGE_IO_H
+#define VBM3D_AMD_IMAGE_IO_H
+
+#include <string>
+#include <vector>
+
+// Returns false and fills `err` on failure.
+bool load_frame(const std::string& path, std::vector<float>& planar, unsigned& w, unsigned& h,
+                unsigned& c, std::string& err);
+bool save_frame(const std::string& path, const std::vector<float>& planar, unsigned w, unsigned h,
+                unsigned c, std::string& err);
+// Two-channel flow, planar [2][y][x] (u then v).
+bool load_flow(const std::string& path, std::vector<float>& planar, unsigned& w, unsigned& h,
+               std::string& err);
+bool save_flow(const std::string& path, const std::vector<float>& planar, unsigned w, unsigned h,
+               std::string& err);
+
+#endif
