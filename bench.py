@@ -99,11 +99,19 @@ def main():
         work.copy_(pristine)
         api.run_device(sigma, work, basic, final, p1, p2, True, args.nb, stream=stream, ctx=ctx)
 
-    for i in range(args.warmup):
+    # the warmup (or an extra untimed step) counts the work per kernel; the
+    # timed steps only bracket launches with events
+    api.profile(2, ctx)
+    for i in range(max(1, args.warmup)):
         one_step()
-        print(f"[bench] warmup {i + 1}/{args.warmup} done", file=sys.stderr, flush=True)
+        print(f"[bench] warmup {i + 1}/{max(1, args.warmup)} done", file=sys.stderr, flush=True)
     torch.cuda.synchronize()
-    api.profile(True, ctx)
+    n_warm = max(1, args.warmup)
+    _, _, bm_cands = api.profile_read("block_match", ctx)
+    _, _, bm_pix = api.profile_read("bm_pixels", ctx)
+    bm_cands /= n_warm
+    bm_pix /= n_warm
+    api.profile(1, ctx)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for i in range(args.steps):
@@ -112,11 +120,10 @@ def main():
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / args.steps
 
-    bm_ms, bm_n, bm_cands = api.profile_read("block_match", ctx)
-    _, _, bm_pix = api.profile_read("bm_pixels", ctx)
+    bm_ms, bm_n, _ = api.profile_read("block_match", ctx)
     kern = {k: api.profile_read(k, ctx)[:2] for k in ("block_match", "group_filter", "aggregate", "colour")}
-    api.profile(False, ctx)
-    flops = 3.0 * bm_pix + 2.0 * bm_cands
+    api.profile(0, ctx)
+    flops = (3.0 * bm_pix + 2.0 * bm_cands) * args.steps
     avg_ms = bm_ms / max(bm_n, 1)
     achieved = flops / max(bm_n, 1) / (avg_ms * 1e-3) / 1e12
 
@@ -141,7 +148,8 @@ def main():
                      "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": None,
                      "avg_launch_ms": round(avg_ms, 3),
-                     "flops_per_launch": flops / max(bm_n, 1)},
+                     "flops_per_launch": flops / max(bm_n, 1),
+                     "candidates_per_step": bm_cands},
     }
     if not args.no_cpu:
         print(f"[bench] cpu baseline on {args.cpu_frames} frames", file=sys.stderr, flush=True)

@@ -268,10 +268,11 @@ def selftest_partial_sort(dist, idx, mid):
 KERNELS = {"block_match": 0, "group_filter": 1, "aggregate": 2, "colour": 3, "bm_pixels": 4}
 
 
-def profile(enable: bool, ctx=None):
-    """Reset and enable (or disable) kernel timing on a context."""
+def profile(mode, ctx=None):
+    """Reset kernel profiling: 0 off, 1 (or True) event timing, 2 timing
+    plus work counters (device atomics: use on an untimed run)."""
     ctx = ctx or default_context()
-    _check(lib().vbm3d_profile(ctx.h, int(bool(enable))))
+    _check(lib().vbm3d_profile(ctx.h, int(mode)))
 
 
 def profile_read(kernel: str, ctx=None):

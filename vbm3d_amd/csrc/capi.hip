@@ -223,7 +223,8 @@ int check(const vbm3d_params& p, unsigned W, unsigned H, unsigned F, unsigned C,
     if (p.Nb == 0 || p.Nb > 8) return no(VBM3D_ERR_UNSUPPORTED, "Nb must be in [1, 8]");
     if (p.Ns == 0 || p.Ns > (unsigned)kMaxWin || p.Npr == 0 || p.Npr > (unsigned)kMaxWin)
         return no(VBM3D_ERR_UNSUPPORTED, "Ns and Npr must be in [1, 8]");
-    if (p.Nb * p.Npr * p.Npr > (unsigned)kMaxSlots) return no(VBM3D_ERR_UNSUPPORTED, "Nb * Npr^2 > 512");
+    if (2 * p.Nb * p.Npr * p.Npr > (unsigned)kMaxSlots) return no(VBM3D_ERR_UNSUPPORTED, "2 Nb Npr^2 > 512");
+    if (p.Nb * p.Nb > 64) return no(VBM3D_ERR_UNSUPPORTED, "Nb^2 > 64");
     if (p.Nb * (2 * p.Nf + 1) > (unsigned)kMaxBest) return no(VBM3D_ERR_UNSUPPORTED, "Nb * (2 Nf + 1) > 64");
     if (p.p == 0) return no(VBM3D_ERR_ARG, "step p must be > 0");
     if (p.T_2D != VBM3D_DCT && p.T_2D != VBM3D_BIOR) return no(VBM3D_ERR_ARG, "T_2D must be DCT (0) or BIOR (1)");
@@ -251,7 +252,8 @@ struct vbm3d_ctx {
     DevBuf idx, nsx, nsim, dist, est, wgt, tabs, refxy, meta, gpos, rlist, scal, stats;
     size_t est_budget = (size_t)32 << 30;
     // kernel profiling (vbm3d_profile / vbm3d_profile_read)
-    bool prof = false;
+    bool prof = false;    // bracket launches with events
+    bool count = false;   // also accumulate work counters (atomics; not for timed runs)
     std::vector<ProfEvent> events;
     double refs_filtered = 0;
     double contributions = 0;
@@ -359,7 +361,7 @@ int run_step(vbm3d_ctx* ctx, const StepIO& io, hipStream_t s) {
     bm.dist_out = nullptr;
     bm.nsim_out = nullptr;
     bm.nsx_out = (uint32_t*)ctx->nsx.p;
-    bm.stats = ctx->prof ? (unsigned long long*)ctx->stats.p : nullptr;
+    bm.stats = ctx->count ? (unsigned long long*)ctx->stats.p : nullptr;
 
     FArgs fa{};
     fa.noisy = io.noisy;
@@ -395,12 +397,12 @@ int run_step(vbm3d_ctx* ctx, const StepIO& io, hipStream_t s) {
         bm.r_end = fa.r_end = (long)e * nrf;
         HIP_TRY(timed(ctx, 0, s, [&] { return launch_bm((int)K, bm, s); }));
         HIP_TRY(timed(ctx, 1, s, [&] { return launch_filter((int)K, io.step, fa, s); }));
-        if (ctx->prof) ctx->refs_filtered += (double)(bm.r_end - bm.r_begin);
+        if (ctx->count) ctx->refs_filtered += (double)(bm.r_end - bm.r_begin);
         const unsigned ready = e >= Fr ? io.F : (e > p.Nf ? e - p.Nf : 0);
         if (ready > t_done) {
             ag.t_begin = t_done;
             HIP_TRY(timed(ctx, 2, s, [&] { return launch_aggregate((int)C, ag, ready - t_done, s); }));
-            if (ctx->prof) ctx->contributions += (double)(ready - t_done) * io.W * io.H * C;
+            if (ctx->count) ctx->contributions += (double)(ready - t_done) * io.W * io.H * C;
             t_done = ready;
         }
     }
@@ -693,7 +695,8 @@ int vbm3d_profile(vbm3d_ctx* ctx, int enable) {
     ctx->events.clear();
     ctx->refs_filtered = ctx->contributions = 0;
     ctx->prof = enable != 0;
-    if (ctx->prof) {
+    ctx->count = enable == 2;
+    if (ctx->count) {
         HIP_TRY(ensure(ctx->stats, 4 * sizeof(unsigned long long)));
         HIP_TRY(hipMemset(ctx->stats.p, 0, 4 * sizeof(unsigned long long)));
     }

@@ -24,6 +24,44 @@ struct Tabs {
     float dct_out;     // 1/(2k)
 };
 
+// Per-wavefront LDS layout of k_bm, in 4-byte words.  Tiles use an odd
+// pitch so that lanes reading neighbouring candidates hit different banks.
+struct BMLay {
+    int th0, tp0, th1, tp1;
+    int ref, tile, sd, si, sv, cd, ci, bd, bi, fd, fi, od, oi, scr, pp, win, total;
+};
+
+__host__ __device__ inline BMLay bm_layout(int K, int C, int Ns, int Npr, int Nb) {
+    BMLay L;
+    int o = 0;
+    L.th0 = Ns + K - 1;
+    L.tp0 = L.th0 | 1;
+    L.th1 = Npr + K - 1;
+    L.tp1 = L.th1 | 1;
+    const int S = ((2 * Nb * Npr * Npr + 127) / 128) * 128;
+    const int t0 = L.th0 * L.tp0 * C, t1 = 2 * Nb * L.th1 * L.tp1 * C;
+    int t = t0 > t1 ? t0 : t1;
+    t = t > 128 ? t : 128;  // the slow-path scratch (64 entries) reuses the tile
+    L.ref = o; o += K * K * C;
+    L.tile = o; o += t;
+    L.scr = L.tile;
+    L.sd = o; o += S;
+    L.si = o; o += S;
+    L.sv = o; o += S;
+    L.cd = o; o += 64;
+    L.ci = o; o += 64;
+    L.bd = o; o += 64;
+    L.bi = o; o += 64;
+    L.fd = o; o += 64;
+    L.fi = o; o += 64;
+    L.od = o; o += 16;
+    L.oi = o; o += 16;
+    L.pp = o; o += 16;
+    L.win = o; o += 80;
+    L.total = o;
+    return L;
+}
+
 struct BMArgs {
     const float* vid;  // matching video [F][C][H][W]
     const float* ff;   // forward flow [F-1][2][H][W] or null

@@ -39,16 +39,30 @@ __device__ __forceinline__ unsigned fkey(float f) {
     return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
 }
 
+// Wave-wide reductions on the DPP network: quad swaps, half-row and row
+// mirrors, then the row-15 / lane-31 broadcasts; lane 63 ends with the
+// result, which readlane makes wave-uniform.
+#define VB_DPP(v, ctrl, rmask) __builtin_amdgcn_update_dpp((int)(v), (int)(v), ctrl, rmask, 0xF, false)
+#define VB_DPP0(v, ctrl, rmask) __builtin_amdgcn_update_dpp(0, (int)(v), ctrl, rmask, 0xF, false)
+
 __device__ __forceinline__ unsigned wave_min_u32(unsigned v) {
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) v = min(v, (unsigned)__shfl_xor((int)v, o, 64));
-    return v;
+    v = min(v, (unsigned)VB_DPP(v, 0xB1, 0xF));   // quad_perm [1,0,3,2]
+    v = min(v, (unsigned)VB_DPP(v, 0x4E, 0xF));   // quad_perm [2,3,0,1]
+    v = min(v, (unsigned)VB_DPP(v, 0x141, 0xF));  // row_half_mirror
+    v = min(v, (unsigned)VB_DPP(v, 0x140, 0xF));  // row_mirror
+    v = min(v, (unsigned)VB_DPP(v, 0x142, 0xA));  // row_bcast15
+    v = min(v, (unsigned)VB_DPP(v, 0x143, 0xC));  // row_bcast31
+    return (unsigned)__builtin_amdgcn_readlane((int)v, 63);
 }
 
 __device__ __forceinline__ int wave_sum_i32(int v) {
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
+    v += VB_DPP0(v, 0xB1, 0xF);
+    v += VB_DPP0(v, 0x4E, 0xF);
+    v += VB_DPP0(v, 0x141, 0xF);
+    v += VB_DPP0(v, 0x140, 0xF);
+    v += VB_DPP0(v, 0x142, 0xA);
+    v += VB_DPP0(v, 0x143, 0xC);
+    return __builtin_amdgcn_readlane(v, 63);
 }
 
 __device__ __forceinline__ unsigned lanes_below(unsigned long long m, int lane) {
@@ -84,49 +98,44 @@ __device__ void wave_select(const float* ld, const uint32_t* li, int m, int mid,
     wave_sync();
 }
 
-// Sum of squared differences over C channels of two KxK patches, in the
-// reference's (c, y, x) order (src/vbm3d.cpp:965-973).
-template <int K, int C>
-__device__ __forceinline__ float patch_ssd(const float* __restrict__ v, unsigned rb,
-                                           unsigned qb, unsigned W, unsigned wh) {
-    float acc = 0.f;
-#pragma unroll
-    for (int c = 0; c < C; ++c)
-#pragma unroll
-        for (int y = 0; y < K; ++y)
-#pragma unroll
-            for (int x = 0; x < K; ++x) {
-                const float dif = v[rb + c * wh + y * W + x] - v[qb + c * wh + y * W + x];
-                acc = acc + dif * dif;
-            }
-    return acc;
-}
-
 // ---------------------------------------------------------------------
 // Predictive block matching: computeSimilarPatches / localSearch /
 // patchDistance (src/vbm3d.cpp:930-1197).
+//
+// One wavefront per reference patch.  Each search stage (the reference
+// frame, then every forward and backward frame) first copies the pixel
+// area its windows cover into a per-wave LDS tile; every lane then forms
+// one candidate's sum of squared differences from LDS against the
+// reference patch (also in LDS), in the reference's (c, y, x) order.
 // ---------------------------------------------------------------------
 template <int K, int C>
 __global__ __launch_bounds__(256) void k_bm(BMArgs a) {
-    __shared__ float s_d[4][kMaxSlots];
-    __shared__ uint32_t s_i[4][kMaxSlots];
-    __shared__ uint8_t s_v[4][kMaxSlots];
-    __shared__ float c_d[4][64];
-    __shared__ uint32_t c_i[4][64];
-    __shared__ float b_d[4][kMaxBest];
-    __shared__ uint32_t b_i[4][kMaxBest];
-    __shared__ float f_d[4][64];
-    __shared__ uint32_t f_i[4][64];
-    __shared__ float o_d[4][64];
-    __shared__ uint32_t o_i[4][64];
-    __shared__ uint32_t pp[4][2][8];  // [0] = backward chain (pre), [1] = forward (post)
-    __shared__ VbEnt scr[4][64];
-
+    extern __shared__ float smem[];
+    const BMLay L = bm_layout(K, C, a.Ns, a.Npr, a.Nb);
     const int lane = threadIdx.x & 63;
     const int w = threadIdx.x >> 6;
-    const long r = a.r_begin + (long)blockIdx.x * 4 + w;
+    const int nw = blockDim.x >> 6;
+    const long r = a.r_begin + (long)blockIdx.x * nw + w;
     if (r >= a.r_end) return;  // wave-uniform
+    float* base = smem + (size_t)w * L.total;
+    float* tile = base + L.tile;
+    float* s_d = base + L.sd;
+    uint32_t* s_i = (uint32_t*)(base + L.si);
+    uint32_t* s_v = (uint32_t*)(base + L.sv);
+    float* c_d = base + L.cd;
+    uint32_t* c_i = (uint32_t*)(base + L.ci);
+    float* b_d = base + L.bd;
+    uint32_t* b_i = (uint32_t*)(base + L.bi);
+    float* f_d = base + L.fd;
+    uint32_t* f_i = (uint32_t*)(base + L.fi);
+    float* o_d = base + L.od;
+    uint32_t* o_i = (uint32_t*)(base + L.oi);
+    VbEnt* scr = (VbEnt*)(base + L.scr);
+    uint32_t* pp = (uint32_t*)(base + L.pp);  // [0..8) backward chain, [8..16) forward
+    int* win = (int*)(base + L.win);          // per window: x0, y0, wx, wy, frame
+
     const unsigned W = a.W, H = a.H, wh = a.wh, whc = a.whc;
+    const float* __restrict__ vid = a.vid;
     unsigned ridx, slot;
     if (a.ref_list) {
         ridx = a.ref_list[r];
@@ -141,11 +150,65 @@ __global__ __launch_bounds__(256) void k_bm(BMArgs a) {
     slot = __builtin_amdgcn_readfirstlane(slot);
     const unsigned rt = ridx / whc, ry = (ridx % wh) / W, rx = ridx % W;
     const float norm = (float)(unsigned)(K * K * a.kt * C);
+    constexpr int K2 = K * K;
 
-    auto dist_of = [&](unsigned qx, unsigned qy, unsigned qt) -> float {
-        float s = patch_ssd<K, C>(a.vid, ridx, qt * whc + qy * W + qx, W, wh);
-        s = s / norm;
-        return s - ((qx == rx && qy == ry) ? a.d : 0.f);
+    // The reference patch is wave-uniform: read it through the constant
+    // address space so it arrives in scalar registers (SMEM loads) and
+    // costs neither LDS bandwidth nor vector issue slots.
+    typedef const __attribute__((address_space(4))) float cfloat;
+    float* refp = base + L.ref;  // the reference patch, [c][y][x]
+    for (int i = lane; i < K2 * C; i += 64) {
+        const int c = i / K2, e = i - c * K2;
+        refp[i] = vid[ridx + c * wh + (e / K) * W + e % K];
+    }
+
+    // SSD of the reference patch against the candidate whose top-left
+    // pixel sits at t[0] of a tile with pitch tp and channel stride cs.
+    auto ssd = [&](const float* t, int tp, int cs) -> float {
+        float acc = 0.f;
+#pragma unroll
+        for (int c = 0; c < C; ++c)
+#pragma unroll
+            for (int y = 0; y < K; ++y) {
+                cfloat* row = (cfloat*)(vid + ridx + c * wh + y * W);
+#pragma unroll
+                for (int x = 0; x < K; ++x) {
+                    const float dif = row[x] - t[c * cs + y * tp + x];
+                    acc = acc + dif * dif;
+                }
+            }
+        return acc;
+    };
+    // two candidates at once: independent accumulators, shared reference
+    // rows; each sum keeps its own sequential order
+    auto ssd2 = [&](const float* t, const float* u, int tp, int cs, float& ra, float& rb) {
+        float acc = 0.f, acd = 0.f;
+#pragma unroll 1
+        for (int c = 0; c < C; ++c)
+#pragma unroll 2
+            for (int y = 0; y < K; ++y) {
+#pragma unroll
+                for (int x = 0; x < K; ++x) {
+                    const float r = refp[(c * K + y) * K + x];
+                    const float dif = r - t[c * cs + y * tp + x];
+                    const float dig = r - u[c * cs + y * tp + x];
+                    acc = acc + dif * dif;
+                    acd = acd + dig * dig;
+                }
+            }
+        ra = acc;
+        rb = acd;
+    };
+    // copy the (wx+K-1) x (wy+K-1) area at (x0, y0) of frame tf into a
+    // tile; lanes cover 4 rows of 16 columns (tile rows are at most 15 wide)
+    const int lrow = lane >> 4, lcol = lane & 15;
+    auto load_tile = [&](float* t, int tp, int cs, int x0, int y0, int wx, int wy, unsigned tf) {
+        const int aw = wx + K - 1, ah = wy + K - 1;
+        const unsigned fb = tf * whc + y0 * W + x0;
+        if (lcol < aw)
+            for (int c = 0; c < C; c++)
+                for (int ty = lrow; ty < ah; ty += 4)
+                    t[c * cs + ty * tp + lcol] = vid[fb + c * wh + ty * W + lcol];
     };
 
     // -- search in the reference frame (src/vbm3d.cpp:1114-1119)
@@ -155,133 +218,177 @@ __global__ __launch_bounds__(256) void k_bm(BMArgs a) {
         const int h = (a.Ns - 1) / 2;
         const int x0 = max(0, (int)rx - h), y0 = max(0, (int)ry - h);
         const int x1 = min((int)W - K, (int)rx + h), y1 = min((int)H - K, (int)ry + h);
-        const int wx = x1 - x0 + 1, m = wx * (y1 - y0 + 1);
+        const int wx = x1 - x0 + 1, wy = y1 - y0 + 1, m = wx * wy;
+        const int cs = L.tp0 * L.th0;
+        load_tile(tile, L.tp0, cs, x0, y0, wx, wy, rt);
+        wave_sync();
         if (lane < m) {
-            const unsigned qx = x0 + lane % wx, qy = y0 + lane / wx;
-            c_d[w][lane] = dist_of(qx, qy, rt);
-            c_i[w][lane] = rt * whc + qy * W + qx;
+            const int lx = lane % wx, ly = lane / wx;
+            const float s = ssd(tile + ly * L.tp0 + lx, L.tp0, cs) / norm;
+            const unsigned qx = x0 + lx, qy = y0 + ly;
+            c_d[lane] = s - ((qx == rx && qy == ry) ? a.d : 0.f);
+            c_i[lane] = rt * whc + qy * W + qx;
         }
         wave_sync();
         n_eval += m;
         const int nb = min(a.Nb, m);
-        wave_select(c_d[w], c_i[w], m, nb, o_d[w], o_i[w], scr[w], lane);
+        wave_select(c_d, c_i, m, nb, o_d, o_i, scr, lane);
         if (lane < nb) {
-            b_d[w][lane] = o_d[w][lane];
-            b_i[w][lane] = o_i[w][lane];
-            pp[w][0][lane] = o_i[w][lane];
-            pp[w][1][lane] = o_i[w][lane];
+            b_d[lane] = o_d[lane];
+            b_i[lane] = o_i[lane];
+            pp[lane] = o_i[lane];
+            pp[8 + lane] = o_i[lane];
         }
         nbest = nb;
         wave_sync();
     }
 
-    // -- predictive search, forward (dir 1) then backward (dir 0)
-    //    (src/vbm3d.cpp:1121-1167)
+    // -- predictive search (src/vbm3d.cpp:1121-1167).  The forward and
+    //    backward chains only depend on themselves, so forward frame f and
+    //    backward frame f are searched in the same pass; their picks are
+    //    stored where the reference's sequential order puts them (all
+    //    forward frames, then all backward frames).
     const int P2 = a.Npr * a.Npr, hs = (a.Npr - 1) / 2;
-    const int nslots = a.Nb * P2;
+    const int cs1 = L.tp1 * L.th1;
     const int nfw = (int)(min(rt + (unsigned)a.Nf, a.F - (unsigned)a.kt) - rt);
     const int nbw = (int)rt - max((int)rt - a.Nf, 0);
-    for (int pass = 0; pass < 2; pass++) {
-        const int dir = pass == 0 ? 1 : 0;
-        const int nst = dir ? nfw : nbw;
-        const float* fl = dir ? a.ff : a.bf;
-        for (int f = 0; f < nst; f++) {
-            // window of temp match wm: flow projection (x first, then y at
-            // the updated x) and clamped Npr x Npr box
-            auto win = [&](int wm, int& x0, int& y0, int& wx, int& wy, unsigned& tf) {
-                const unsigned mi = pp[w][dir][wm];
+    const int nst = max(nfw, nbw);
+    const int Nb = a.Nb;
+    for (int f = 0; f < nst; f++) {
+        const bool act[2] = {f < nfw, f < nbw};  // [0] forward, [1] backward
+        // windows: flow projection of each temp match (x first, then y at
+        // the updated x), clamped Npr x Npr box in the next frame
+        if (lane < 2 * Nb) {
+            const int ch = lane / Nb, m = lane - ch * Nb;
+            if (act[ch]) {
+                const int dir = ch == 0 ? 1 : -1;
+                const float* fl = ch == 0 ? a.ff : a.bf;
+                const unsigned mi = pp[(ch == 0 ? 8 : 0) + m];
                 unsigned px = mi % W, py = (mi % wh) / W;
                 const unsigned pt = mi / whc;
-                const unsigned ft = dir ? pt : pt - 1;
+                const unsigned ft = ch == 0 ? pt : pt - 1;
                 const unsigned hk = K / 2;
                 const float fx = fl ? fl[ft * 2 * wh + (py + hk) * W + px + hk] : 0.f;
                 px = (unsigned)min(max((int)((float)px + roundf(fx)), 0), (int)(W - K));
                 const float fy = fl ? fl[ft * 2 * wh + wh + (py + hk) * W + px + hk] : 0.f;
                 py = (unsigned)min(max((int)((float)py + roundf(fy)), 0), (int)(H - K));
-                tf = dir ? pt + 1 : pt - 1;
-                x0 = max(0, (int)px - hs);
-                y0 = max(0, (int)py - hs);
-                wx = min((int)W - K, (int)px + hs) - x0 + 1;
-                wy = min((int)H - K, (int)py + hs) - y0 + 1;
-            };
-            for (int base = 0; base < nslots; base += 64) {
-                const int s = base + lane;
+                const int x0 = max(0, (int)px - hs), y0 = max(0, (int)py - hs);
+                win[lane * 5 + 0] = x0;
+                win[lane * 5 + 1] = y0;
+                win[lane * 5 + 2] = min((int)W - K, (int)px + hs) - x0 + 1;
+                win[lane * 5 + 3] = min((int)H - K, (int)py + hs) - y0 + 1;
+                win[lane * 5 + 4] = (int)(pt + dir);
+            }
+        }
+        wave_sync();
+        for (int wn = 0; wn < 2 * Nb; wn++)
+            if (act[wn / Nb])
+                load_tile(tile + wn * C * cs1, L.tp1, cs1, win[wn * 5], win[wn * 5 + 1], win[wn * 5 + 2],
+                          win[wn * 5 + 3], (unsigned)win[wn * 5 + 4]);
+        wave_sync();
+        // slots: chain ch, window m, candidate l -> (ch * Nb + m) * P2 + l;
+        // lane evaluates slots b0 + lane and b0 + 64 + lane together
+        const int nslots = 2 * Nb * P2;
+        for (int b0 = 0; b0 < nslots; b0 += 128) {
+            bool valid[2] = {false, false};
+            int toff[2] = {0, 0}, qxs[2] = {0, 0}, qys[2] = {0, 0};
+            unsigned tfs[2] = {0, 0};
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                const int s = b0 + 64 * h + lane;
                 if (s < nslots) {
-                    const int wm = s / P2, l = s - wm * P2;
-                    int x0, y0, wx, wy;
-                    unsigned tf;
-                    win(wm, x0, y0, wx, wy, tf);
-                    bool valid = false;
-                    float dv = 0.f;
-                    unsigned iv = 0;
-                    if (l < wx * wy) {
-                        const int qx = x0 + l % wx, qy = y0 + l / wx;
-                        // already seen = inside an earlier window of this frame
-                        bool seen = false;
-                        for (int m2 = 0; m2 < wm; m2++) {
-                            int ax, ay, aw, ah;
-                            unsigned at;
-                            win(m2, ax, ay, aw, ah, at);
-                            seen |= qx >= ax && qx < ax + aw && qy >= ay && qy < ay + ah;
-                        }
-                        if (!seen) {
-                            valid = true;
-                            dv = dist_of(qx, qy, tf);
-                            iv = tf * whc + qy * W + qx;
+                    const int wn = s / P2, l = s - wn * P2, ch = wn / Nb, m = wn - ch * Nb;
+                    if (act[ch]) {
+                        const int x0 = win[wn * 5], y0 = win[wn * 5 + 1], wx = win[wn * 5 + 2], wy = win[wn * 5 + 3];
+                        if (l < wx * wy) {
+                            const int lx = l % wx, ly = l / wx;
+                            const int qx = x0 + lx, qy = y0 + ly;
+                            // already seen = inside an earlier window of this frame
+                            bool seen = false;
+                            for (int m2 = 0; m2 < m; m2++) {
+                                const int* w2 = win + (ch * Nb + m2) * 5;
+                                seen |= qx >= w2[0] && qx < w2[0] + w2[2] && qy >= w2[1] && qy < w2[1] + w2[3];
+                            }
+                            valid[h] = !seen;
+                            toff[h] = wn * C * cs1 + ly * L.tp1 + lx;
+                            qxs[h] = qx;
+                            qys[h] = qy;
+                            tfs[h] = (unsigned)win[wn * 5 + 4];
                         }
                     }
-                    s_d[w][s] = dv;
-                    s_i[w][s] = iv;
-                    s_v[w][s] = valid;
                 }
             }
-            wave_sync();
+            if (valid[0] || valid[1]) {
+                float sa, sb;
+                ssd2(tile + (valid[0] ? toff[0] : toff[1]), tile + (valid[1] ? toff[1] : toff[0]), L.tp1, cs1, sa, sb);
+                const float sv[2] = {sa, sb};
+#pragma unroll
+                for (int h = 0; h < 2; h++) {
+                    const int s = b0 + 64 * h + lane;
+                    if (valid[h]) {
+                        s_d[s] = sv[h] / norm - (((unsigned)qxs[h] == rx && (unsigned)qys[h] == ry) ? a.d : 0.f);
+                        s_i[s] = tfs[h] * whc + qys[h] * W + qxs[h];
+                    }
+                }
+            }
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                const int s = b0 + 64 * h + lane;
+                if (s < nslots) s_v[s] = valid[h];
+            }
+        }
+        wave_sync();
+        for (int ch = 0; ch < 2; ch++) {
+            if (!act[ch]) continue;
             // top Nb of each window's list, appended in window order
             int nfb = 0;
-            for (int wm = 0; wm < a.Nb; wm++) {
-                const bool v = lane < P2 && s_v[w][wm * P2 + lane];
+            for (int m = 0; m < Nb; m++) {
+                const int sb = (ch * Nb + m) * P2;
+                const bool v = lane < P2 && s_v[sb + lane];
                 const unsigned long long bal = __ballot(v);
                 if (v) {
                     const unsigned pos = lanes_below(bal, lane);
-                    c_d[w][pos] = s_d[w][wm * P2 + lane];
-                    c_i[w][pos] = s_i[w][wm * P2 + lane];
+                    c_d[pos] = s_d[sb + lane];
+                    c_i[pos] = s_i[sb + lane];
                 }
-                const int m = __popcll(bal);
-                n_eval += m;
+                const int mm = __popcll(bal);
+                n_eval += mm;
                 wave_sync();
-                const int nb = min(a.Nb, m);
-                wave_select(c_d[w], c_i[w], m, nb, o_d[w], o_i[w], scr[w], lane);
+                const int nb = min(Nb, mm);
+                wave_select(c_d, c_i, mm, nb, o_d, o_i, scr, lane);
                 if (lane < nb) {
-                    f_d[w][nfb + lane] = o_d[w][lane];
-                    f_i[w][nfb + lane] = o_i[w][lane];
+                    f_d[nfb + lane] = o_d[lane];
+                    f_i[nfb + lane] = o_i[lane];
                 }
                 nfb += nb;
                 wave_sync();
             }
-            const int nb = min(a.Nb, nfb);
-            wave_select(f_d[w], f_i[w], nfb, nb, o_d[w], o_i[w], scr[w], lane);
-            if (lane < nb) {
-                pp[w][dir][lane] = o_i[w][lane];
-                b_d[w][nbest + lane] = o_d[w][lane];
-                b_i[w][nbest + lane] = o_i[w][lane];
+            // each frame contributes exactly Nb picks (window 0 alone holds
+            // Nb candidates), so the slot of frame f in `best` is fixed
+            wave_select(f_d, f_i, nfb, Nb, o_d, o_i, scr, lane);
+            const int at = Nb * (1 + (ch == 0 ? f : nfw + f));
+            if (lane < Nb) {
+                pp[(ch == 0 ? 8 : 0) + lane] = o_i[lane];
+                b_d[at + lane] = o_d[lane];
+                b_i[at + lane] = o_i[lane];
             }
-            nbest += nb;
             wave_sync();
         }
     }
+    nbest = Nb * (1 + nfw + nbw);
 
     // -- N best overall, tau cut, power-of-two group size
     //    (src/vbm3d.cpp:1169-1184)
     const int nsim = min(a.N, nbest);
-    wave_select(b_d[w], b_i[w], nbest, nsim, o_d[w], o_i[w], scr[w], lane);
+    wave_select(b_d, b_i, nbest, nsim, o_d, o_i, scr, lane);
     int it = nsim - 1;
-    while (o_d[w][it] > a.tau && it > 0) it--;
+    while (o_d[it] > a.tau && it > 0) it--;
     unsigned nsx = 1;
     while (nsx * 2 <= (unsigned)(it + 1)) nsx *= 2;
     if (lane < a.N) {
-        a.idx_out[(size_t)slot * a.N + lane] = lane < nsim ? o_i[w][lane] : 0xFFFFFFFFu;
+        a.idx_out[(size_t)slot * a.N + lane] = lane < nsim ? o_i[lane] : 0xFFFFFFFFu;
         if (a.dist_out)
-            a.dist_out[(size_t)slot * a.N + lane] = lane < nsim ? o_d[w][lane] : __builtin_nanf("");
+            a.dist_out[(size_t)slot * a.N + lane] = lane < nsim ? o_d[lane] : __builtin_nanf("");
     }
     if (lane == 0) {
         a.nsx_out[slot] = nsx;
@@ -507,7 +614,7 @@ __device__ float shrink(float* zg, float* eg, float* v2, int lane, bool had, flo
         float acc = 0.f;
         if (lane == 0)
             for (int f = 0; f < K2 * NS; f++) acc = acc + v2[f];
-        acc = __shfl(acc, 0, 64);
+        acc = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(acc), 0));
         t3_inv<NS>(e, had);
         if (had) weight = acc > 0.0f ? 1.0f / (s2 * acc) : 1.0f;
         else weight = 1.f / (s2 * acc);
@@ -757,7 +864,13 @@ template <int K, int C>
 static hipError_t bm_k(const BMArgs& a, hipStream_t s) {
     const long n = a.r_end - a.r_begin;
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL((k_bm<K, C>), dim3((unsigned)((n + 3) / 4)), dim3(256), 0, s, a);
+    const BMLay L = bm_layout(K, C, a.Ns, a.Npr, a.Nb);
+    const size_t per_wave = (size_t)L.total * 4;
+    int waves = 4;
+    while (waves > 1 && per_wave * waves > 64 * 1024) waves--;
+    if (per_wave > 64 * 1024) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((k_bm<K, C>), dim3((unsigned)((n + waves - 1) / waves)), dim3(64 * waves),
+                       per_wave * waves, s, a);
     return hipGetLastError();
 }
 
