@@ -474,94 +474,12 @@ __device__ __forceinline__ void t3_inv(float* v, bool had) {
     else haar_inv<NS>(v);
 }
 
-// Forward 2D transform of the KxK patch in xs; returns coefficient `lane`
-// (bior_2d_forward src/lib_transforms.cpp:125-181, or the DCT with
-// coef_norm src/vbm3d.cpp:1245-1302).
-template <int K>
-__device__ float fwd2d(const float* xs, float* t2, int lane, bool bior, const Tabs& T) {
-    constexpr int K2 = K * K;
-    float out = 0.f;
-    if (bior) {
-        const int i = lane >> 3, j = lane & 7;
-        float m = 0.f;
-#pragma unroll
-        for (int s = 0; s < 8; s++) m = m + xs[i * 8 + s] * T.biorF[j * 8 + s];
-        t2[lane] = m;
-        wave_sync();
-        m = 0.f;
-#pragma unroll
-        for (int s = 0; s < 8; s++) m = m + t2[s * 8 + i] * T.biorF[j * 8 + s];
-        out = m;
-    } else {
-        const int p = lane / K, v = lane % K;
-        if (lane < K2) {
-            float m = 0.f;
-#pragma unroll
-            for (int q = 0; q < K; q++) m = m + xs[p * K + q] * T.dctF[v * K + q];
-            t2[lane] = m;
-        }
-        wave_sync();
-        if (lane < K2) {
-            float m = 0.f;
-#pragma unroll
-            for (int pp = 0; pp < K; pp++) m = m + t2[pp * K + v] * T.dctF[p * K + pp];
-            out = m * T.nrm[lane];
-        }
-    }
-    wave_sync();
-    return out;
-}
-
-// Inverse 2D transform of the coefficient vector held one per lane;
-// returns the value of pixel *pix (bior_2d_inverse
-// src/lib_transforms.cpp:270-321, dct_2d_inv src/vbm3d.cpp:1678-1713).
-template <int K>
-__device__ float inv2d(float cv, float* xs, float* t2, int lane, bool bior, const Tabs& T,
-                       int* pix) {
-    constexpr int K2 = K * K;
-    float out = 0.f;
-    if (bior) {
-        xs[lane] = cv;
-        wave_sync();
-        const int i = lane >> 3, j = lane & 7;
-        float m = 0.f;
-#pragma unroll
-        for (int s = 0; s < 8; s++) m = m + xs[i + s * 8] * T.biorI[j * 8 + s];
-        t2[lane] = m;
-        wave_sync();
-        m = 0.f;
-#pragma unroll
-        for (int s = 0; s < 8; s++) m = m + t2[s * 8 + i] * T.biorI[j * 8 + s];
-        *pix = j * 8 + i;
-        out = m;
-    } else {
-        if (lane < K2) xs[lane] = cv * T.inv[lane];
-        wave_sync();
-        const int p = lane / K, q = lane % K;
-        if (lane < K2) {
-            float m = 0.f;
-#pragma unroll
-            for (int v = 0; v < K; v++) m = m + xs[p * K + v] * T.dctI[q * K + v];
-            t2[lane] = m;
-        }
-        wave_sync();
-        if (lane < K2) {
-            float m = 0.f;
-#pragma unroll
-            for (int u = 0; u < K; u++) m = m + t2[u * K + q] * T.dctI[p * K + u];
-            out = T.dct_out * m;
-        }
-        *pix = lane;
-    }
-    wave_sync();
-    return out;
-}
-
-// 1D transform along the group, shrinkage and inverse for one channel
+// 1D transform along the group and shrinkage for one channel
 // (ht_filtering_haar/_hadamard src/vbm3d.cpp:1405-1521,
-// wiener_filtering_haar/_hadamard :1539-1666).  z (and e) hold lane kk's
-// coefficient for every group member, in LDS [n][64].  Returns the
-// aggregation weight of the channel (valid in every lane).
+// wiener_filtering_haar/_hadamard :1539-1666).  Lane kk holds coefficient
+// kk of every group member: zg/eg are [n][64].  Step 1 returns the
+// aggregation weight (valid in every lane); step 2 stores value^2 at the
+// flat index n + kk*nSx of v2 for the weight sum done afterwards.
 template <int K, int NS, int STEP>
 __device__ float shrink(float* zg, float* eg, float* v2, int lane, bool had, float lambda,
                         float sig) {
@@ -570,10 +488,10 @@ __device__ float shrink(float* zg, float* eg, float* v2, int lane, bool had, flo
     float z[NS], e[NS];
 #pragma unroll
     for (int n = 0; n < NS; n++) {
-        z[n] = act ? zg[n * 64 + lane] : 0.f;
-        if (STEP == 2) e[n] = act ? eg[n * 64 + lane] : 0.f;
+        z[n] = act ? zg[n * K2 + lane] : 0.f;
+        if (STEP == 2) e[n] = act ? eg[n * K2 + lane] : 0.f;
     }
-    float weight;
+    float weight = 0.f;
     if (STEP == 1) {
         t3_fwd<NS>(z, had);
         const float thr = had ? lambda * sig * sqrtf((float)NS) : lambda * sig;
@@ -594,7 +512,7 @@ __device__ float shrink(float* zg, float* eg, float* v2, int lane, bool had, flo
         weight = 1.f / (sig * sig * (float)cnt);
 #pragma unroll
         for (int n = 0; n < NS; n++)
-            if (act) zg[n * 64 + lane] = z[n];
+            if (act) zg[n * K2 + lane] = z[n];
     } else {
         t3_fwd<NS>(z, had);
         t3_fwd<NS>(e, had);
@@ -606,85 +524,178 @@ __device__ float shrink(float* zg, float* eg, float* v2, int lane, bool had, flo
             if (had) value = value * coef;
             value /= (value + s2);
             e[n] = had ? z[n] * value * coef : z[n] * value;
-            if (act) v2[lane * NS + n] = value * value;  // flat index n + kk*nSx
+            if (act) v2[lane * NS + n] = value * value;
         }
-        wave_sync();
-        // Wiener weight: sequential float sum over the flat index
-        // (src/vbm3d.cpp:1651), one lane
-        float acc = 0.f;
-        if (lane == 0)
-            for (int f = 0; f < K2 * NS; f++) acc = acc + v2[f];
-        acc = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(acc), 0));
         t3_inv<NS>(e, had);
-        if (had) weight = acc > 0.0f ? 1.0f / (s2 * acc) : 1.0f;
-        else weight = 1.f / (s2 * acc);
 #pragma unroll
         for (int n = 0; n < NS; n++)
-            if (act) eg[n * 64 + lane] = e[n];
-        wave_sync();
+            if (act) eg[n * K2 + lane] = e[n];
     }
     return weight;
+}
+
+// Transposition buffer: patch n of a lane batch is a K x (K+1) tile at
+// n*xstride(K) (row pitch K+1).  The strides keep both the row-wise write and
+// the column-wise read of the 64 lanes (n, r) = n*K + r bank-conflict free
+// at K = 8 and within 2-way at K = 7.
+__host__ __device__ constexpr int xstride(int K) { return (K & 1) ? K * (K + 1) + 1 : K * (K + 1); }
+
+// Per-wavefront LDS layout of k_filter (4-byte words).
+__host__ __device__ inline int filter_lds_words(int K, int C, int N, int step) {
+    const int pb = 64 / K;
+    int w = pb * xstride(K) + N * K * K;
+    if (step == 2) w += N * K * K + C * K * K * N;
+    return w;
 }
 
 template <int K, int C, int STEP>
 __global__ __launch_bounds__(256) void k_filter(FArgs a) {
     constexpr int K2 = K * K;
-    __shared__ float xs[4][64], t2[4][64];
-    __shared__ float zg[4][kMaxN * 64];
-    __shared__ float eg[4][STEP == 2 ? kMaxN * 64 : 1];
-    __shared__ float v2[4][STEP == 2 ? kMaxN * 64 : 1];
+    constexpr int PB = 64 / K;           // patches per lane batch
+    constexpr int XS = xstride(K), XP = K + 1;
+    extern __shared__ float smem[];
     const int lane = threadIdx.x & 63;
     const int w = threadIdx.x >> 6;
-    const long r = a.r_begin + (long)blockIdx.x * 4 + w;
+    const int nw = blockDim.x >> 6;
+    const long r = a.r_begin + (long)blockIdx.x * nw + w;
     if (r >= a.r_end) return;
+    const int words = filter_lds_words(K, C, a.N, STEP);
+    float* X = smem + (size_t)w * words;          // [PB][XS] transposes
+    float* zg = X + PB * XS;                       // [N][K2]
+    float* eg = zg + a.N * K2;                     // [N][K2] (step 2)
+    float* v2 = eg + a.N * K2;                     // [C][K2*N] (step 2)
     const unsigned t = (unsigned)(r / a.nrf), i = (unsigned)(r - (long)t * a.nrf);
     const unsigned slot = __builtin_amdgcn_readfirstlane((t % a.ring) * a.nrf + i);
     const unsigned W = a.W, wh = a.wh, whc = a.whc;
     const int nsx = (int)__builtin_amdgcn_readfirstlane(a.nsx[slot]);
-    const Tabs& T = *a.tabs;
     const bool bior = a.T2D == 1;
     const bool had = a.T3D == 2;
-    const int pr = lane / K, pc = lane % K;
     const uint32_t* mem = a.idx + (size_t)slot * a.N;
+    typedef const __attribute__((address_space(4))) float cfloat;
+    const cfloat* TF = (const cfloat*)(bior ? a.tabs->biorF : a.tabs->dctF);
+    const cfloat* TI = (const cfloat*)(bior ? a.tabs->biorI : a.tabs->dctI);
+    const cfloat* NRM = (const cfloat*)a.tabs->nrm;
+    const cfloat* INV = (const cfloat*)a.tabs->inv;
+    const float dct_out = a.tabs->dct_out;
+    const int nl = lane / K, rr = lane - nl * K;    // batch patch, row / column index
+    const bool lact = lane < PB * K;
+    float weight[C];
 
     for (int c = 0; c < C; c++) {
-        // 2D transforms of every group member (table_2D[c][n][kk])
-        for (int n = 0; n < nsx; n++) {
-            const unsigned mi = mem[n];
-            const unsigned px = mi % W, py = (mi % wh) / W, pt = mi / whc;
-            const unsigned base = pt * whc + c * wh + (py + pr) * W + px + pc;
-            if (lane < K2) xs[w][lane] = a.noisy[base];
-            wave_sync();
-            const float zc = fwd2d<K>(xs[w], t2[w], lane, bior, T);
-            if (lane < K2) zg[w][n * 64 + lane] = zc;
-            if (STEP == 2) {
-                if (lane < K2) xs[w][lane] = a.basic[base];
+        // ---- forward 2D transforms of all members (bior_2d_forward
+        //      src/lib_transforms.cpp:125-181; DCT src/vbm3d.cpp:1245-1302)
+        for (int g = 0; g < (STEP == 2 ? 2 : 1); g++) {
+            const float* src = g == 0 ? a.noisy : a.basic;
+            float* dst = g == 0 ? zg : eg;
+            for (int nb0 = 0; nb0 < nsx; nb0 += PB) {
+                const int n = nb0 + nl;
+                const bool act = lact && n < nsx;
+                if (act) {
+                    const unsigned mi = mem[n];
+                    const unsigned px = mi % W, py = (mi % wh) / W, pt = mi / whc;
+                    const float* row = src + pt * whc + c * wh + (py + rr) * W + px;
+                    float x[K];
+#pragma unroll
+                    for (int q = 0; q < K; q++) x[q] = row[q];
+                    // pass 1 along the row: t[v] = sum_q x[q] M[v][q]
+#pragma unroll
+                    for (int v = 0; v < K; v++) {
+                        float m = 0.f;
+#pragma unroll
+                        for (int q = 0; q < K; q++) m = m + x[q] * TF[v * K + q];
+                        X[nl * XS + v * XP + rr] = m;  // stored transposed
+                    }
+                }
                 wave_sync();
-                const float ec = fwd2d<K>(xs[w], t2[w], lane, bior, T);
-                if (lane < K2) eg[w][n * 64 + lane] = ec;
+                if (act) {
+                    // pass 2 down column rr of t
+                    float col[K];
+#pragma unroll
+                    for (int p = 0; p < K; p++) col[p] = X[nl * XS + rr * XP + p];
+#pragma unroll
+                    for (int u = 0; u < K; u++) {
+                        float m = 0.f;
+#pragma unroll
+                        for (int p = 0; p < K; p++) m = m + col[p] * TF[u * K + p];
+                        // BIOR keeps out[i*8+j] = (column i, output j); the
+                        // DCT writes out[u*K+v] scaled by coef_norm
+                        if (bior) dst[n * K2 + rr * 8 + u] = m;
+                        else dst[n * K2 + u * K + rr] = m * NRM[u * K + rr];
+                    }
+                }
+                wave_sync();
             }
         }
-        wave_sync();
-        float* est_grp = STEP == 2 ? eg[w] : zg[w];
-        float weight;
+        // ---- 1D transform along the group, shrinkage, inverse
+        float* v2c = v2 + c * K2 * a.N;
+        float wc;
         switch (nsx) {
-            case 1: weight = shrink<K, 1, STEP>(zg[w], eg[w], v2[w], lane, had, a.lambda, a.sig[c]); break;
-            case 2: weight = shrink<K, 2, STEP>(zg[w], eg[w], v2[w], lane, had, a.lambda, a.sig[c]); break;
-            case 4: weight = shrink<K, 4, STEP>(zg[w], eg[w], v2[w], lane, had, a.lambda, a.sig[c]); break;
-            case 8: weight = shrink<K, 8, STEP>(zg[w], eg[w], v2[w], lane, had, a.lambda, a.sig[c]); break;
-            default: weight = shrink<K, 16, STEP>(zg[w], eg[w], v2[w], lane, had, a.lambda, a.sig[c]); break;
+            case 1: wc = shrink<K, 1, STEP>(zg, eg, v2c, lane, had, a.lambda, a.sig[c]); break;
+            case 2: wc = shrink<K, 2, STEP>(zg, eg, v2c, lane, had, a.lambda, a.sig[c]); break;
+            case 4: wc = shrink<K, 4, STEP>(zg, eg, v2c, lane, had, a.lambda, a.sig[c]); break;
+            case 8: wc = shrink<K, 8, STEP>(zg, eg, v2c, lane, had, a.lambda, a.sig[c]); break;
+            default: wc = shrink<K, 16, STEP>(zg, eg, v2c, lane, had, a.lambda, a.sig[c]); break;
         }
+        weight[c] = wc;
         wave_sync();
-        // inverse 2D per member and store [slot][c][n][pixel]
-        float* dst = a.est + (((size_t)slot * C + c) * a.N) * K2;
-        for (int n = 0; n < nsx; n++) {
-            const float cv = lane < K2 ? est_grp[n * 64 + lane] : 0.f;
-            int pix = 0;
-            const float val = inv2d<K>(cv, xs[w], t2[w], lane, bior, T, &pix);
-            if (lane < K2) dst[(size_t)n * K2 + pix] = val;
+        // ---- inverse 2D transforms (bior_2d_inverse src/lib_transforms.cpp:
+        //      270-321; dct_2d_inv src/vbm3d.cpp:1678-1713), store [n][pixel]
+        const float* est = STEP == 2 ? eg : zg;
+        float* dstg = a.est + (((size_t)slot * C + c) * a.N) * K2;
+        for (int nb0 = 0; nb0 < nsx; nb0 += PB) {
+            const int n = nb0 + nl;
+            const bool act = lact && n < nsx;
+            if (act) {
+                float x[K];
+#pragma unroll
+                for (int s2 = 0; s2 < K; s2++)
+                    x[s2] = bior ? est[n * K2 + s2 * 8 + rr] : est[n * K2 + rr * K + s2] * INV[rr * K + s2];
+#pragma unroll
+                for (int v = 0; v < K; v++) {
+                    float m = 0.f;
+#pragma unroll
+                    for (int q = 0; q < K; q++) m = m + x[q] * TI[v * K + q];
+                    X[nl * XS + v * XP + rr] = m;
+                }
+            }
+            wave_sync();
+            if (act) {
+                float col[K];
+#pragma unroll
+                for (int p = 0; p < K; p++) col[p] = X[nl * XS + rr * XP + p];
+#pragma unroll
+                for (int u = 0; u < K; u++) {
+                    float m = 0.f;
+#pragma unroll
+                    for (int p = 0; p < K; p++) m = m + col[p] * TI[u * K + p];
+                    // BIOR: pixel (row u, column rr); DCT: pixel (row u,
+                    // column rr) after the 1/(2k) factor
+                    dstg[(size_t)n * K2 + u * K + rr] = bior ? m : dct_out * m;
+                }
+            }
+            wave_sync();
         }
-        if (lane == 0) a.wgt[(size_t)slot * C + c] = weight;
+    }
+    if (STEP == 2) {
+        // Wiener weights: sequential float sums over the flat index
+        // (src/vbm3d.cpp:1651), one lane per channel
         wave_sync();
+        if (lane < C) {
+            const float* vc = v2 + lane * K2 * a.N;
+            float acc = 0.f;
+            for (int f = 0; f < K2 * nsx; f++) acc = acc + vc[f];
+            const float sg = a.sig[lane], s2 = sg * sg;
+            float wgt;
+            if (had) wgt = acc > 0.0f ? 1.0f / (s2 * acc) : 1.0f;
+            else wgt = 1.f / (s2 * acc);
+            a.wgt[(size_t)slot * C + lane] = wgt;
+        }
+    } else if (lane < C) {
+        float wsel = weight[0];
+#pragma unroll
+        for (int c = 1; c < C; c++)
+            if (lane == c) wsel = weight[c];
+        a.wgt[(size_t)slot * C + lane] = wsel;
     }
 }
 
@@ -889,7 +900,12 @@ template <int K, int C, int STEP>
 static hipError_t filter_k(const FArgs& a, hipStream_t s) {
     const long n = a.r_end - a.r_begin;
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL((k_filter<K, C, STEP>), dim3((unsigned)((n + 3) / 4)), dim3(256), 0, s, a);
+    const size_t per_wave = (size_t)filter_lds_words(K, C, a.N, STEP) * 4;
+    int waves = 4;
+    while (waves > 1 && per_wave * waves > 64 * 1024) waves--;
+    if (per_wave > 64 * 1024) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((k_filter<K, C, STEP>), dim3((unsigned)((n + waves - 1) / waves)),
+                       dim3(64 * waves), per_wave * waves, s, a);
     return hipGetLastError();
 }
 
