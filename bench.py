@@ -5,15 +5,20 @@ One step = one full run_vbm3d (colour transform, step 1 hard-threshold
 basic estimate, step 2 Wiener final estimate, inverse colour) over the
 whole synthetic clip, inputs already resident in HBM.  Default workload:
 60 RGB frames 1920x1080, sigma 20, synthetic generator (velocities up to
-4 px/frame, seed 2).  `--sigma 40` gives the sigma-40 variant.
+4 px/frame, seed 2).  The same clip at sigma 40 is also timed and
+reported under `config.sigma40`.
 
-Prints one JSON line.  `roofline` is the block-matching kernel (the
-dominant one): algorithmic FP32 operations per launch (3 per pixel
-comparison -- sub, mul, add -- plus 2 per candidate: divide, bias) over
-its HIP-event-timed average launch duration, against MI355X's FP32 peak.
-`cpu_baseline` is the CPU oracle (an op-order-faithful C++/OpenMP
-restatement of the reference path) timed on this host on a sample of
-the same workload.
+Prints one JSON line.  `roofline` is the dominant kernel, the block
+matching of the step that spends the most time in it: algorithmic FP32
+operations per launch (3 per pixel comparison -- sub, mul, add -- plus 2
+per candidate: divide, bias) over its HIP-event-timed average launch
+duration, against MI355X's FP32 peak.  Its `traffic` (HBM bytes per
+launch) comes from the rocprofv3 FETCH_SIZE / WRITE_SIZE summary in
+profiles/traffic.json when one matches the workload.  `hbm` is the whole
+job's algorithmic HBM rate (20*C bytes per output pixel) against the HBM
+peak.  `cpu_baseline` is the CPU oracle (an op-order-faithful C++/OpenMP
+restatement of the reference path) timed on this host on a sample of the
+same workload.
 """
 from __future__ import annotations
 
@@ -70,30 +75,65 @@ def main():
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--vmax", type=int, default=4)
     ap.add_argument("--nb", type=int, default=1, help="reference grid (1 = single-thread grid)")
-    ap.add_argument("--cpu-frames", type=int, default=6)
+    ap.add_argument("--cpu-frames", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-sigma40", action="store_true", help="skip the sigma-40 line")
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
-              "(the per-clip path shards by clips or temporal slabs, see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py: one GPU per process; --gpus {args.gpus} is not supported "
+              "(a multi-GPU run would shard by independent clips or temporal slabs)", file=sys.stderr)
         sys.exit(2)
 
     import torch
     from vbm3d_amd import api, synth
 
     W, H, F, C, sigma = args.width, args.height, args.frames, args.channels, args.sigma
+    dev = torch.device("cuda:0")
+    ctx = api.Context(0)
+    stream = torch.cuda.current_stream()
+    res = measure(api, synth, torch, ctx, stream, dev, args, sigma)
+    if not args.no_sigma40 and sigma != 40.0:
+        s40 = measure(api, synth, torch, ctx, stream, dev, args, 40.0)
+        res["config"]["sigma40"] = {k: s40[k] for k in ("value", "unit", "ms_per_step")}
+        res["config"]["sigma40"].update({k: s40["config"][k] for k in ("psnr_noisy_db", "psnr_final_db")})
+        res["config"]["sigma40"]["bm_roofline_frac"] = s40["roofline"]["frac"]
+    if not args.no_cpu:
+        print(f"[bench] cpu baseline on {args.cpu_frames} frames", file=sys.stderr, flush=True)
+        _, noisy = synth.noisy_clip(W, H, min(args.cpu_frames, F), C, sigma, args.seed, vmax=args.vmax)
+        res["cpu_baseline"] = cpu_baseline(noisy, sigma, min(args.cpu_frames, F))
+    print(json.dumps(res), flush=True)
+
+
+def kernel_name(prm, C):
+    return f"k_bm<{prm.k}, {C}>"
+
+
+def load_traffic(workload: str, kernel: str):
+    """Per-launch HBM bytes of `kernel` for `workload` from the committed
+    rocprofv3 summary (FETCH_SIZE and WRITE_SIZE runs, calibrated)."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            tab = json.load(f)
+    except (OSError, ValueError):
+        return None
+    ent = tab.get(workload, {}).get(kernel)
+    if not ent:
+        return None
+    return ent["fetch_bytes_per_launch"] + ent["write_bytes_per_launch"]
+
+
+def measure(api, synth, torch, ctx, stream, dev, args, sigma):
+    W, H, F, C = args.width, args.height, args.frames, args.channels
     print(f"[bench] generating {F}x{C}x{H}x{W} sigma={sigma:g}", file=sys.stderr, flush=True)
     clean, noisy = synth.noisy_clip(W, H, F, C, sigma, args.seed, vmax=args.vmax)
-    dev = torch.device("cuda:0")
     pristine = torch.from_numpy(noisy).to(dev)
     work = torch.empty_like(pristine)
     basic = torch.empty_like(pristine)
     final = torch.empty_like(pristine)
     p1 = api.initialize_parameters_1(sigma)
     p2 = api.initialize_parameters_2(sigma)
-    ctx = api.Context(0)
-    stream = torch.cuda.current_stream()
 
     def one_step():
         work.copy_(pristine)
@@ -101,16 +141,14 @@ def main():
 
     # the warmup (or an extra untimed step) counts the work per kernel; the
     # timed steps only bracket launches with events
-    api.profile(2, ctx)
-    for i in range(max(1, args.warmup)):
-        one_step()
-        print(f"[bench] warmup {i + 1}/{max(1, args.warmup)} done", file=sys.stderr, flush=True)
-    torch.cuda.synchronize()
     n_warm = max(1, args.warmup)
-    _, _, bm_cands = api.profile_read("block_match", ctx)
-    _, _, bm_pix = api.profile_read("bm_pixels", ctx)
-    bm_cands /= n_warm
-    bm_pix /= n_warm
+    api.profile(2, ctx)
+    for i in range(n_warm):
+        one_step()
+        print(f"[bench] warmup {i + 1}/{n_warm} done", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    cands = [api.profile_read("block_match", ctx, st)[2] / n_warm for st in (1, 2)]
+    pix = [api.profile_read("bm_pixels", ctx, st)[2] / n_warm for st in (1, 2)]
     api.profile(1, ctx)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -120,41 +158,54 @@ def main():
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / args.steps
 
-    bm_ms, bm_n, _ = api.profile_read("block_match", ctx)
-    kern = {k: api.profile_read(k, ctx)[:2] for k in ("block_match", "group_filter", "aggregate", "colour")}
+    names = ("block_match", "group_filter", "aggregate", "colour")
+    kern = {}
+    for st in (1, 2):
+        for k in names[:3]:
+            ms, n, _ = api.profile_read(k, ctx, st)
+            kern[f"step{st}_{k}"] = (ms, n)
+    kern["colour"] = api.profile_read("colour", ctx)[:2]
     api.profile(0, ctx)
-    flops = (3.0 * bm_pix + 2.0 * bm_cands) * args.steps
+    # dominant kernel: the block matching of the step that spends longest in it
+    dom = max((1, 2), key=lambda st: kern[f"step{st}_block_match"][0])
+    bm_ms, bm_n = kern[f"step{dom}_block_match"]
+    prm = p1 if dom == 1 else p2
+    flops_launch = (3.0 * pix[dom - 1] + 2.0 * cands[dom - 1]) * args.steps / max(bm_n, 1)
     avg_ms = bm_ms / max(bm_n, 1)
-    achieved = flops / max(bm_n, 1) / (avg_ms * 1e-3) / 1e12
+    achieved = flops_launch / (avg_ms * 1e-3) / 1e12
 
     out = final.cpu().numpy()
     psnr_final = synth.psnr(clean, out)
     psnr_noisy = synth.psnr(clean, noisy)
-
     mpix = W * H * F / dt / 1e6
-    res = {
+    workload = (f"{F} {'RGB' if C == 3 else 'gray'} frames {W}x{H}, sigma={sigma:g}, both steps, "
+                f"default parameters, reference grid nb={args.nb}")
+    kname = kernel_name(prm, C)
+    hbm_gbs = 20.0 * C * W * H * F / dt / 1e9
+    return {
         "metric": "denoised Mpixel/s, 1080p RGB sigma=20, PSNR within 0.01 dB of reference",
         "value": round(mpix, 3), "unit": "Mpixel/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(dt * 1e3, 2), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
-        "config": {"workload": f"{F} {'RGB' if C == 3 else 'gray'} frames {W}x{H}, sigma={sigma:g}, "
-                               f"both steps, default parameters, reference grid nb={args.nb}",
+        "config": {"workload": workload,
                    "frames": F, "width": W, "height": H, "channels": C, "sigma": sigma,
                    "seed": args.seed, "vmax": args.vmax,
                    "psnr_noisy_db": round(psnr_noisy, 3), "psnr_final_db": round(psnr_final, 3),
                    "kernel_ms_per_step": {k: round(v[0] / args.steps, 2) for k, v in kern.items()},
                    "kernel_launches_per_step": {k: v[1] // args.steps for k, v in kern.items()}},
-        "roofline": {"bound": "mfma", "kernel": "k_bm (block matching)",
+        "roofline": {"bound": "mfma", "kernel": f"{kname} (block matching, step {dom})",
                      "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": None,
+                     "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
+                     "traffic": load_traffic(workload, kname),
+                     "peak_note": "FP32 vector peak (the MFMA FP32 figure is the same 157.3); "
+                                  "the kernel runs on VALU without FMA contraction or packed FP32",
                      "avg_launch_ms": round(avg_ms, 3),
-                     "flops_per_launch": flops / max(bm_n, 1),
-                     "candidates_per_step": bm_cands},
+                     "flops_per_launch": flops_launch,
+                     "candidates_per_step": cands[dom - 1]},
+        "hbm": {"achieved": round(hbm_gbs, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(hbm_gbs / HBM_PEAK_GBS, 6),
+                "bytes_per_pixel": 20 * C},
     }
-    if not args.no_cpu:
-        print(f"[bench] cpu baseline on {args.cpu_frames} frames", file=sys.stderr, flush=True)
-        res["cpu_baseline"] = cpu_baseline(noisy, sigma, min(args.cpu_frames, F))
-    print(json.dumps(res), flush=True)
 
 
 if __name__ == "__main__":

@@ -136,7 +136,9 @@ long vbm3d_grid(unsigned W, unsigned H, unsigned k, unsigned p, int n,
  * 2 (aggregation) or 3 (colour transform): total milliseconds, number of
  * launches and the work done -- candidate patches evaluated (0), group
  * members filtered (1), output samples produced (2).  kernel 4 reads the
- * block-matching pixel comparisons (candidates * k*k*C; no time). */
+ * block-matching pixel comparisons (candidates * k*k*C; no time).  Adding
+ * 8 (step 1) or 16 (step 2) to the code restricts the figures to one
+ * step; the plain code sums both. */
 int vbm3d_profile(vbm3d_ctx* ctx, int enable);
 int vbm3d_profile_read(vbm3d_ctx* ctx, int kernel, double* ms, long* launches,
                        double* work);

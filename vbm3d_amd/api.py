@@ -275,9 +275,13 @@ def profile(mode, ctx=None):
     _check(lib().vbm3d_profile(ctx.h, int(mode)))
 
 
-def profile_read(kernel: str, ctx=None):
-    """(total_ms, launches, work) for one kernel class since profile(True)."""
+def profile_read(kernel: str, ctx=None, step: int = 0):
+    """(total_ms, launches, work) for one kernel class since profile(True);
+    step 1 or 2 restricts the figures to that step, 0 sums both."""
     ctx = ctx or default_context()
+    if step not in (0, 1, 2):
+        raise ValueError("step must be 0, 1 or 2")
     ms, n, w = C.c_double(), C.c_long(), C.c_double()
-    _check(lib().vbm3d_profile_read(ctx.h, KERNELS[kernel], C.byref(ms), C.byref(n), C.byref(w)))
+    code = KERNELS[kernel] + 8 * step
+    _check(lib().vbm3d_profile_read(ctx.h, code, C.byref(ms), C.byref(n), C.byref(w)))
     return ms.value, n.value, w.value

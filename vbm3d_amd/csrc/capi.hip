@@ -242,6 +242,7 @@ int check(const vbm3d_params& p, unsigned W, unsigned H, unsigned F, unsigned C,
 
 struct ProfEvent {
     int kernel;
+    int step;  // 1 or 2; 0 for the colour transforms
     hipEvent_t a, b;
 };
 
@@ -249,14 +250,14 @@ struct vbm3d_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     DevBuf noisy, basic, den, ff, bf, num_raw, den_raw;  // host-buffer entry points
-    DevBuf idx, nsx, nsim, dist, est, wgt, tabs, refxy, meta, gpos, rlist, scal, stats;
+    DevBuf idx, nsx, nsim, dist, est, wgt, tabs, refxy, meta, gpos, rlist, scal, stats, counts;
     size_t est_budget = (size_t)32 << 30;
     // kernel profiling (vbm3d_profile / vbm3d_profile_read)
     bool prof = false;    // bracket launches with events
     bool count = false;   // also accumulate work counters (atomics; not for timed runs)
     std::vector<ProfEvent> events;
-    double refs_filtered = 0;
-    double contributions = 0;
+    double contributions[3] = {0, 0, 0};  // output samples, per step
+    double pix_per_cand[3] = {0, 0, 0};   // k*k*C of each step's matching
 };
 
 namespace {
@@ -281,9 +282,9 @@ struct StepIO {
 
 // Brackets one launch with events when profiling is on.
 template <class F>
-hipError_t timed(vbm3d_ctx* ctx, int kernel, hipStream_t s, F&& launch) {
+hipError_t timed(vbm3d_ctx* ctx, int kernel, int step, hipStream_t s, F&& launch) {
     if (!ctx->prof) return launch();
-    ProfEvent ev{kernel, nullptr, nullptr};
+    ProfEvent ev{kernel, step, nullptr, nullptr};
     hipError_t e = hipEventCreate(&ev.a);
     if (e == hipSuccess) e = hipEventCreate(&ev.b);
     if (e == hipSuccess) e = hipEventRecord(ev.a, s);
@@ -361,7 +362,12 @@ int run_step(vbm3d_ctx* ctx, const StepIO& io, hipStream_t s) {
     bm.dist_out = nullptr;
     bm.nsim_out = nullptr;
     bm.nsx_out = (uint32_t*)ctx->nsx.p;
-    bm.stats = ctx->count ? (unsigned long long*)ctx->stats.p : nullptr;
+    bm.count_out = nullptr;
+    if (ctx->count) {
+        HIP_TRY(ensure(ctx->counts, (size_t)S * nrf * 2 * 4));
+        bm.count_out = (uint32_t*)ctx->counts.p;
+        ctx->pix_per_cand[io.step] = (double)K2 * C;
+    }
 
     FArgs fa{};
     fa.noisy = io.noisy;
@@ -385,6 +391,7 @@ int run_step(vbm3d_ctx* ctx, const StepIO& io, hipStream_t s) {
     ag.idx = bm.idx_out; ag.nsx = bm.nsx_out;
     ag.est = fa.est; ag.wgt = fa.wgt; ag.tabs = fa.tabs;
     ag.R = (int)((p.Ns - 1) / 2 + p.Nf * ((p.Npr - 1) / 2 + io.maxflow));
+    ag.divW = make_div32(io.W);
     ag.fallback = io.fallback;
     ag.out = io.out;
     ag.num_out = io.num_raw;
@@ -395,14 +402,16 @@ int run_step(vbm3d_ctx* ctx, const StepIO& io, hipStream_t s) {
         const unsigned e = std::min(Fr, a + S);
         bm.r_begin = fa.r_begin = (long)a * nrf;
         bm.r_end = fa.r_end = (long)e * nrf;
-        HIP_TRY(timed(ctx, 0, s, [&] { return launch_bm((int)K, bm, s); }));
-        HIP_TRY(timed(ctx, 1, s, [&] { return launch_filter((int)K, io.step, fa, s); }));
-        if (ctx->count) ctx->refs_filtered += (double)(bm.r_end - bm.r_begin);
+        HIP_TRY(timed(ctx, 0, io.step, s, [&] { return launch_bm((int)K, bm, s); }));
+        if (ctx->count)
+            HIP_TRY(launch_sum_counts(bm.count_out, bm.r_end - bm.r_begin,
+                                      (unsigned long long*)ctx->stats.p + 4 * (io.step - 1), s));
+        HIP_TRY(timed(ctx, 1, io.step, s, [&] { return launch_filter((int)K, io.step, fa, s); }));
         const unsigned ready = e >= Fr ? io.F : (e > p.Nf ? e - p.Nf : 0);
         if (ready > t_done) {
             ag.t_begin = t_done;
-            HIP_TRY(timed(ctx, 2, s, [&] { return launch_aggregate((int)C, ag, ready - t_done, s); }));
-            if (ctx->count) ctx->contributions += (double)(ready - t_done) * io.W * io.H * C;
+            HIP_TRY(timed(ctx, 2, io.step, s, [&] { return launch_aggregate((int)C, ag, ready - t_done, s); }));
+            if (ctx->count) ctx->contributions[io.step] += (double)(ready - t_done) * io.W * io.H * C;
             t_done = ready;
         }
     }
@@ -435,7 +444,7 @@ int run_device(vbm3d_ctx* ctx, float sigma, float* noisy, const float* ff, const
     int rc = flow_absmax(ctx, ff, bf, W, H, F, s, &maxflow);
     if (rc) return rc;
     if (cs && C == 3)
-        HIP_TRY(timed(ctx, 3, s, [&] { return launch_color(p1->k > 0 ? noisy : basic, W, H, F, true, s); }));
+        HIP_TRY(timed(ctx, 3, 0, s, [&] { return launch_color(p1->k > 0 ? noisy : basic, W, H, F, true, s); }));
     StepIO io{};
     io.sigma = sigma; io.noisy = noisy; io.ff = ff; io.bf = bf;
     io.W = W; io.H = H; io.F = F; io.C = C; io.cs = cs != 0; io.nb = nb; io.maxflow = maxflow;
@@ -454,9 +463,9 @@ int run_device(vbm3d_ctx* ctx, float sigma, float* noisy, const float* ff, const
         HIP_TRY(hipMemcpyAsync(den, basic, (size_t)W * H * F * C * 4, hipMemcpyDeviceToDevice, s));
     }
     if (cs && C == 3) {
-        HIP_TRY(timed(ctx, 3, s, [&] { return launch_color(den, W, H, F, false, s); }));
-        HIP_TRY(timed(ctx, 3, s, [&] { return launch_color(noisy, W, H, F, false, s); }));
-        HIP_TRY(timed(ctx, 3, s, [&] { return launch_color(basic, W, H, F, false, s); }));
+        HIP_TRY(timed(ctx, 3, 0, s, [&] { return launch_color(den, W, H, F, false, s); }));
+        HIP_TRY(timed(ctx, 3, 0, s, [&] { return launch_color(noisy, W, H, F, false, s); }));
+        HIP_TRY(timed(ctx, 3, 0, s, [&] { return launch_color(basic, W, H, F, false, s); }));
     }
     HIP_TRY(hipStreamSynchronize(s));
     return VBM3D_OK;
@@ -693,12 +702,12 @@ int vbm3d_profile(vbm3d_ctx* ctx, int enable) {
         if (e.b) (void)hipEventDestroy(e.b);
     }
     ctx->events.clear();
-    ctx->refs_filtered = ctx->contributions = 0;
+    for (int k = 0; k < 3; k++) ctx->contributions[k] = ctx->pix_per_cand[k] = 0;
     ctx->prof = enable != 0;
     ctx->count = enable == 2;
     if (ctx->count) {
-        HIP_TRY(ensure(ctx->stats, 4 * sizeof(unsigned long long)));
-        HIP_TRY(hipMemset(ctx->stats.p, 0, 4 * sizeof(unsigned long long)));
+        HIP_TRY(ensure(ctx->stats, 8 * sizeof(unsigned long long)));
+        HIP_TRY(hipMemset(ctx->stats.p, 0, 8 * sizeof(unsigned long long)));
     }
     return VBM3D_OK;
 }
@@ -706,22 +715,31 @@ int vbm3d_profile(vbm3d_ctx* ctx, int enable) {
 int vbm3d_profile_read(vbm3d_ctx* ctx, int kernel, double* ms, long* launches, double* work) {
     if (!ctx || !ms || !launches || !work) return fail(VBM3D_ERR_ARG, "null argument");
     HIP_TRY(hipSetDevice(ctx->device));
+    const int base = kernel & 7, step = kernel >> 3;
+    if (base > 4 || step > 2) return fail(VBM3D_ERR_ARG, "unknown kernel code");
     for (ProfEvent& e : ctx->events) HIP_TRY(hipEventSynchronize(e.b));
     double tot = 0;
     long n = 0;
     for (ProfEvent& e : ctx->events)
-        if (e.kernel == kernel) {
+        if (e.kernel == base && (step == 0 || e.step == step)) {
             float t = 0;
             HIP_TRY(hipEventElapsedTime(&t, e.a, e.b));
             tot += t;
             n++;
         }
-    unsigned long long st[4] = {0, 0, 0, 0};
-    if (ctx->stats.p) HIP_TRY(hipMemcpy(st, ctx->stats.p, sizeof st, hipMemcpyDeviceToHost));
+    unsigned long long st[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (ctx->stats.p && ctx->count)
+        HIP_TRY(hipMemcpy(st, ctx->stats.p, sizeof st, hipMemcpyDeviceToHost));
+    double w = 0;
+    for (int k = 1; k <= 2; k++) {
+        if (step != 0 && step != k) continue;
+        const unsigned long long* sk = st + 4 * (k - 1);
+        w += base == 0 ? (double)sk[0] : base == 1 ? (double)sk[1]
+           : base == 2 ? ctx->contributions[k] : base == 4 ? (double)sk[0] * ctx->pix_per_cand[k] : 0.0;
+    }
     *ms = tot;
     *launches = n;
-    *work = kernel == 0 ? (double)st[0] : kernel == 1 ? (double)st[1]
-          : kernel == 2 ? ctx->contributions : kernel == 4 ? (double)st[2] : 0.0;
+    *work = w;
     return VBM3D_OK;
 }
 

@@ -78,8 +78,8 @@ struct BMArgs {
     float* dist_out;     // [slot][N] or null
     uint32_t* nsim_out;  // [slot] or null
     uint32_t* nsx_out;   // [slot]
-    unsigned long long* stats;  // null, or [0] += candidates evaluated, [1] += group members,
-                                // [2] += pixel comparisons (candidates * K*K*C)
+    uint32_t* count_out;  // null, or per patch of the launch, at 2*(r - r_begin):
+                          // candidates evaluated, group size
 };
 
 struct FArgs {
@@ -98,6 +98,27 @@ struct FArgs {
     float* wgt;  // [slot][C]
 };
 
+// Exact unsigned 32-bit division by a run-time constant d >= 1 with a
+// multiply-high and two shifts (Granlund and Montgomery, "Division by
+// invariant integers using multiplication", PLDI 1994, round-up variant):
+// l = ceil(log2 d), m = floor(2^32 (2^l - d) / d) + 1,
+// n / d = (t + ((n - t) >> min(l, 1))) >> max(l - 1, 0), t = mulhi(m, n).
+struct Div32 {
+    uint32_t m, s1, s2;
+};
+
+__host__ __device__ inline Div32 make_div32(uint32_t d) {
+    uint32_t l = 0;
+    while (((uint64_t)1 << l) < d) l++;
+    const uint64_t m = (((((uint64_t)1) << l) - d) << 32) / d + 1;
+    return Div32{(uint32_t)m, l ? 1u : 0u, l ? l - 1 : 0u};
+}
+
+__host__ __device__ inline uint32_t div32(uint32_t n, Div32 v) {
+    const uint32_t t = (uint32_t)(((uint64_t)v.m * n) >> 32);
+    return (t + ((n - t) >> v.s1)) >> v.s2;
+}
+
 struct AArgs {
     unsigned W, H, F, C, wh, whc;
     int K, N, Nf, kt;
@@ -111,6 +132,7 @@ struct AArgs {
     const float* wgt;
     const Tabs* tabs;
     int R;                      // largest member-to-reference displacement
+    Div32 divW;                 // division by W
     unsigned t_begin;           // first output frame of this launch
     const float* fallback;      // used where den == 0
     float* out;                 // out = den == 0 ? fallback : num / den
@@ -124,6 +146,7 @@ hipError_t launch_filter(int K, int step, const FArgs& a, hipStream_t s);
 hipError_t launch_aggregate(int C, const AArgs& a, unsigned frames, hipStream_t s);
 hipError_t launch_color(float* v, unsigned W, unsigned H, unsigned F, bool forward,
                         hipStream_t s);
+hipError_t launch_sum_counts(const uint32_t* counts, long n, unsigned long long* stats, hipStream_t s);
 hipError_t launch_flow_absmax(const float* fl, size_t n, unsigned* out, hipStream_t s);
 
 }  // namespace vb

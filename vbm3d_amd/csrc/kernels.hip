@@ -393,10 +393,9 @@ __global__ __launch_bounds__(256) void k_bm(BMArgs a) {
     if (lane == 0) {
         a.nsx_out[slot] = nsx;
         if (a.nsim_out) a.nsim_out[slot] = nsim;
-        if (a.stats) {
-            atomicAdd(&a.stats[0], (unsigned long long)n_eval);
-            atomicAdd(&a.stats[1], (unsigned long long)nsx);
-            atomicAdd(&a.stats[2], (unsigned long long)n_eval * (K * K * C));
+        if (a.count_out) {
+            a.count_out[2 * (r - a.r_begin)] = (uint32_t)n_eval;
+            a.count_out[2 * (r - a.r_begin) + 1] = nsx;
         }
     }
 }
@@ -713,26 +712,41 @@ __device__ __forceinline__ int lower_bound_u16(const uint16_t* a, int n, int v) 
     return lo;
 }
 
+// One 16x16 output tile of one frame per workgroup.  The reference patches
+// whose groups can reach the tile are scanned in processing order (crop,
+// frame, row, column), 256 at a time; the group members that overlap the
+// tile are compacted into an ordered list, and each wavefront then
+// accumulates, for its 8x8 quadrant, the members that overlap the quadrant
+// in list order: every output sample sees its contributions in exactly the
+// reference's order.
 template <int C>
 __global__ __launch_bounds__(256) void k_aggregate(AArgs a) {
     constexpr int TS = kAggTile;
+    constexpr int QS = TS / 2;
     __shared__ uint32_t L_sn[256 * kMaxN];
     __shared__ uint32_t L_pos[256 * kMaxN];
+    __shared__ float s_kai[kMaxWin * kMaxWin * 4];
     __shared__ int s_w[4];
     const int tid = threadIdx.x;
     const int lane = tid & 63, wv = tid >> 6;
     const int x0 = blockIdx.x * TS, y0 = blockIdx.y * TS;
     const unsigned to = a.t_begin + blockIdx.z;
-    const int px = x0 + (tid % TS), py = y0 + (tid / TS);
+    const int qx0 = x0 + (wv & 1) * QS, qy0 = y0 + (wv >> 1) * QS;  // this wave's quadrant
+    const int px = qx0 + (lane & (QS - 1)), py = qy0 + lane / QS;
     const bool inside = px < (int)a.W && py < (int)a.H;
     const int K = a.K, K2 = K * K, N = a.N;
     const unsigned W = a.W, wh = a.wh, whc = a.whc;
+    for (int i = tid; i < K2; i += 256) s_kai[i] = a.tabs->kaiser[i];
     float tn[C], td[C];
 #pragma unroll
     for (int c = 0; c < C; c++) tn[c] = td[c] = 0.f;
     const int trlo = max(0, (int)to - a.Nf), trhi = min((int)a.Fr - 1, (int)to + a.Nf);
     const int rlo = y0 - a.R - (K - 1), rhi = y0 + TS - 1 + a.R;
     const int clo = x0 - a.R - (K - 1), chi = x0 + TS - 1 + a.R;
+    // a member at q = to*whc + y*W + x overlaps the tile's rows iff
+    // q - to*whc lies in [ylo*W, yhi*W)
+    const unsigned tbase = to * whc;
+    const unsigned rmin = (unsigned)max(0, y0 - K + 1) * W, rmax = (unsigned)min((int)a.H, y0 + TS) * W;
 
     for (int ci = 0; ci < a.ncrop; ci++) {
         const int* meta = a.crop_meta + ci * 5;
@@ -743,27 +757,26 @@ __global__ __launch_bounds__(256) void k_aggregate(AArgs a) {
         const int ja = lower_bound_u16(cols, ncols, clo), jb = lower_bound_u16(cols, ncols, chi + 1);
         const int nr = ib - ia, nc = jb - ja, ntr = trhi - trlo + 1;
         const long total = (nr > 0 && nc > 0 && ntr > 0) ? (long)ntr * nr * nc : 0;
+        const Div32 dnc = make_div32(nc > 0 ? nc : 1), dnr = make_div32(nr > 0 ? nr : 1);
         float pn[C], pd[C];
 #pragma unroll
         for (int c = 0; c < C; c++) pn[c] = pd[c] = 0.f;
 
         for (long cb = 0; cb < total; cb += 256) {
-            const long e = cb + tid;
+            const unsigned e = (unsigned)(cb + tid);
             uint32_t hits = 0, slot = 0;
             if (e < total) {
-                const int jj = (int)(e % nc);
-                const long rest = e / nc;
-                const int ii = (int)(rest % nr);
-                const int tr = trlo + (int)(rest / nr);
-                slot = ((unsigned)tr % a.ring) * a.nrf + off + (ia + ii) * ncols + (ja + jj);
+                const unsigned rest = div32(e, dnc), jj = e - rest * nc;
+                const unsigned tq = div32(rest, dnr), ii = rest - tq * nr;
+                const unsigned tr = trlo + tq;
+                slot = (tr % a.ring) * a.nrf + off + (ia + ii) * ncols + (ja + jj);
                 const int ns = (int)a.nsx[slot];
                 const uint32_t* mem = a.idx + (size_t)slot * N;
                 for (int n = 0; n < ns; n++) {
-                    const unsigned mi = mem[n];
-                    if (mi / whc != to) continue;
-                    const int qy = (int)((mi % wh) / W), qx = (int)(mi % W);
-                    if (qx <= x0 + TS - 1 && qx + K - 1 >= x0 && qy <= y0 + TS - 1 && qy + K - 1 >= y0)
-                        hits |= 1u << n;
+                    const unsigned r = mem[n] - tbase;
+                    if (r < rmin || r >= rmax) continue;
+                    const int qy = (int)div32(r, a.divW), qx = (int)(r - (unsigned)qy * W);
+                    if (qx <= x0 + TS - 1 && qx + K - 1 >= x0) hits |= 1u << n;
                 }
             }
             // ordered compaction: exclusive block scan of the hit counts
@@ -782,25 +795,36 @@ __global__ __launch_bounds__(256) void k_aggregate(AArgs a) {
             int o = base + x - cnt;
             for (int n = 0; n < N; n++)
                 if ((hits >> n) & 1u) {
-                    const unsigned mi = a.idx[(size_t)slot * N + n];
+                    const unsigned r = a.idx[(size_t)slot * N + n] - tbase;
+                    const unsigned qy = div32(r, a.divW), qx = r - qy * W;
                     L_sn[o] = slot * kMaxN + n;
-                    L_pos[o] = (mi % W) | (((mi % wh) / W) << 16);
+                    L_pos[o] = qx | (qy << 16);
                     o++;
                 }
             __syncthreads();
-            if (inside) {
-                for (int j = 0; j < tot; j++) {
+            // this wave: the list entries overlapping its quadrant, in order
+            for (int j0 = 0; j0 < tot; j0 += 64) {
+                bool ov = false;
+                if (j0 + lane < tot) {
+                    const uint32_t pos = L_pos[j0 + lane];
+                    const int ex = (int)(pos & 0xFFFFu), ey = (int)(pos >> 16);
+                    ov = ex <= qx0 + QS - 1 && ex + K - 1 >= qx0 && ey <= qy0 + QS - 1 && ey + K - 1 >= qy0;
+                }
+                uint64_t m = __ballot(ov);
+                while (m) {
+                    const int j = j0 + (int)__builtin_ctzll(m);
+                    m &= m - 1;
                     const uint32_t pos = L_pos[j];
                     const int dx = px - (int)(pos & 0xFFFFu), dy = py - (int)(pos >> 16);
-                    if ((unsigned)dx < (unsigned)K && (unsigned)dy < (unsigned)K) {
+                    if (inside && (unsigned)dx < (unsigned)K && (unsigned)dy < (unsigned)K) {
                         const uint32_t sn = L_sn[j];
-                        const uint32_t s = sn / kMaxN, n = sn % kMaxN;
+                        const uint32_t sl = sn / kMaxN, n = sn % kMaxN;
                         const int kk = dy * K + dx;
-                        const float kai = a.tabs->kaiser[kk];
+                        const float kai = s_kai[kk];
 #pragma unroll
                         for (int c = 0; c < C; c++) {
-                            const float kw = kai * a.wgt[(size_t)s * C + c];
-                            pn[c] = pn[c] + kw * a.est[(((size_t)s * C + c) * N + n) * K2 + kk];
+                            const float kw = kai * a.wgt[(size_t)sl * C + c];
+                            pn[c] = pn[c] + kw * a.est[(((size_t)sl * C + c) * N + n) * K2 + kk];
                             pd[c] = pd[c] + kw;
                         }
                     }
@@ -854,6 +878,31 @@ __global__ void k_color(float* v, unsigned wh, unsigned F, int forward) {
     r[k] = o0;
     r[wh + k] = o1;
     r[2 * wh + k] = o2;
+}
+
+// Work counters of one block-matching launch: stats[0] += candidates,
+// stats[1] += group members (one atomic per block).
+__global__ void k_sum_counts(const uint32_t* cnt, long n, unsigned long long* stats) {
+    __shared__ unsigned long long part[2][4];
+    unsigned long long c0 = 0, c1 = 0;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        c0 += cnt[2 * i];
+        c1 += cnt[2 * i + 1];
+    }
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        c0 += (unsigned long long)__shfl_xor((long long)c0, o, 64);
+        c1 += (unsigned long long)__shfl_xor((long long)c1, o, 64);
+    }
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) { part[0][w] = c0; part[1][w] = c1; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long s0 = 0, s1 = 0;
+        for (int k = 0; k < (int)(blockDim.x >> 6); k++) { s0 += part[0][k]; s1 += part[1][k]; }
+        atomicAdd(&stats[0], s0);
+        atomicAdd(&stats[1], s1);
+    }
 }
 
 __global__ void k_flow_absmax(const float* fl, size_t n, unsigned* out) {
@@ -940,6 +989,12 @@ hipError_t launch_color(float* v, unsigned W, unsigned H, unsigned F, bool forwa
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(k_color, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, v,
                        W * H, F, forward ? 1 : 0);
+    return hipGetLastError();
+}
+
+hipError_t launch_sum_counts(const uint32_t* counts, long n, unsigned long long* stats, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_sum_counts, dim3(256), dim3(256), 0, s, counts, n, stats);
     return hipGetLastError();
 }
 
