@@ -130,7 +130,8 @@ long vbm3d_grid(unsigned W, unsigned H, unsigned k, unsigned p, int n,
  * the reference, for roofline measurements).  vbm3d_profile(ctx, 1)
  * resets the counters and brackets every later launch with HIP events on
  * the stream it runs on; vbm3d_profile(ctx, 2) does the same and also
- * counts the work done (with device atomics, so not for timed runs);
+ * counts the work done (an extra store per patch and a reduction launch
+ * after each block-matching launch, so not for timed runs);
  * vbm3d_profile(ctx, 0) stops.  vbm3d_profile_read
  * returns, for kernel 0 (block matching), 1 (group filtering),
  * 2 (aggregation) or 3 (colour transform): total milliseconds, number of
