@@ -120,6 +120,23 @@ int vbm3d_step(vbm3d_ctx* ctx, int step, float sigma, const float* noisy,
                const vbm3d_params* prm, int color_space, int ref_grid_nb,
                float* num, float* den);
 
+/* Adjoint of step 2 (no counterpart in the reference): with the basic
+ * estimate fixed, the groups, Wiener gains and aggregation weights are
+ * frozen and step 2 is linear in the noisy video.  Computes
+ * d_grad_noisy = J^T d_grad_out, J the Jacobian of the step-2 output
+ * (num/den with the den == 0 fallback of vbm3d_run) with respect to the
+ * colour-transformed noisy video.  The groups are recomputed from
+ * d_basic (block matching runs twice) rather than stored.  d_final, if
+ * not NULL, receives the forward output.  Device buffers, [F][C][H][W];
+ * enqueued on `stream` (NULL = the context's stream) and synchronised. */
+int vbm3d_step2_adjoint(vbm3d_ctx* ctx, float sigma, const float* d_noisy,
+                        const float* d_basic, const float* d_fflow,
+                        const float* d_bflow, unsigned W, unsigned H,
+                        unsigned F, unsigned C, const vbm3d_params* p2,
+                        int color_space, int ref_grid_nb,
+                        const float* d_grad_out, float* d_grad_noisy,
+                        float* d_final, void* stream);
+
 /* Reference-patch grid of one frame in processing order (crop, row, col);
  * ind_initialize + subDivideTight (src/Utilities/Utilities.cpp:400-415,
  * src/Utilities/LibVideoT.hpp:903-961).  Returns the count, or -1. */

@@ -68,6 +68,8 @@ def lib():
                                C.c_uint, pp, C.c_int, C.c_int, C.c_int, fp, fp]
         L.orc_step_fp64.argtypes = [C.c_int, C.c_float, fp, fp, fp, fp, C.c_uint, C.c_uint, C.c_uint,
                                     C.c_uint, pp, C.c_int, dp, dp]
+        L.orc_step2_adjoint_fp64.argtypes = [C.c_float, fp, fp, fp, fp, C.c_uint, C.c_uint, C.c_uint,
+                                             C.c_uint, pp, C.c_int, C.c_int, dp, dp, dp]
         L.orc_bior_fwd.argtypes = [fp, fp]
         L.orc_bior_inv.argtypes = [fp]
         L.orc_dct_fwd.argtypes = [C.c_uint, fp, fp]
@@ -186,6 +188,22 @@ def step_fp64(stepno, sigma, noisy, basic, prm, color_space=True):
     _chk(lib().orc_step_fp64(stepno, float(sigma), _f(nz), bs, None, None, W, H, F, Cc, C.byref(prm),
                              int(bool(color_space)), _d(num), _d(den)))
     return num, den
+
+
+def step2_adjoint_fp64(sigma, noisy, basic, prm, grad_out, color_space=True, nb=1):
+    """(grad_noisy, final) in float64: the transpose of step 2's Jacobian
+    (groups, Wiener gains and weights frozen by the basic estimate) applied
+    to grad_out, and the FP64 forward value."""
+    prm = as_orc(prm)
+    nz = np.ascontiguousarray(noisy, np.float32)
+    bs = np.ascontiguousarray(basic, np.float32)
+    F, Cc, H, W = nz.shape
+    g = np.ascontiguousarray(grad_out, np.float64)
+    gz = np.empty(nz.shape, np.float64)
+    fin = np.empty(nz.shape, np.float64)
+    _chk(lib().orc_step2_adjoint_fp64(float(sigma), _f(nz), _f(bs), None, None, W, H, F, Cc, C.byref(prm),
+                                      int(bool(color_space)), nb, _d(g), _d(gz), _d(fin)))
+    return gz, fin
 
 
 def bior_fwd(patch):

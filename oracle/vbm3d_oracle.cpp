@@ -972,6 +972,149 @@ int run_step(StepIn& in, int nb, int nthreads, T* num, T* den, bool replay,
     return 0;
 }
 
+// ---------------------------------------------------------------------
+// Adjoint of step 2 with the groups, the Wiener gains and the weights
+// frozen (FP64).  With the basic estimate fixed, step 2 is linear in the
+// noisy video:  final = D^-1 sum_g Agg_g (kaiser w_g) G T3^-1 V_g T3 F Gather_g z,
+// D = den.  Its transpose gathers u = grad/den with kaiser w_g, applies
+// G^T, T3^T V_g T3^-T (= T3 V_g T3^-1 for the orthonormal Haar and the
+// symmetric Hadamard) and F^T, and scatters back to the member patches.
+// ---------------------------------------------------------------------
+// Transposes of the 2D transforms as explicit double sums.  Patches are
+// row-major x[p][q]; coefficient layouts are the forward transforms'.
+void dct_fwd_T(const double* c, double* y, const TabT<double>& tb) {
+    // forward: out = nrm o (F x F^T)  =>  x' = F^T (nrm o c) F
+    const unsigned k = tb.k;
+    for (unsigned p = 0; p < k; p++)
+        for (unsigned q = 0; q < k; q++) {
+            double m = 0;
+            for (unsigned u = 0; u < k; u++)
+                for (unsigned v = 0; v < k; v++)
+                    m += tb.dctF[u * k + p] * tb.nrm[u * k + v] * c[u * k + v] * tb.dctF[v * k + q];
+            y[p * k + q] = m;
+        }
+}
+
+void dct_inv_T(const double* r, double* y, const TabT<double>& tb) {
+    // inverse: pix = dct_out (I (c o inv) I^T)  =>  c' = inv o dct_out (I^T r I)
+    const unsigned k = tb.k;
+    for (unsigned u = 0; u < k; u++)
+        for (unsigned v = 0; v < k; v++) {
+            double m = 0;
+            for (unsigned p = 0; p < k; p++)
+                for (unsigned q = 0; q < k; q++)
+                    m += tb.dctI[p * k + u] * r[p * k + q] * tb.dctI[q * k + v];
+            y[u * k + v] = tb.inv[u * k + v] * tb.dct_out * m;
+        }
+}
+
+void bior_fwd_T(const double* c, double* y, const TabT<double>& tb) {
+    // forward: out = B x^T B^T (B = analysis)  =>  x'[s][q] = sum c[i][j] B[j][s] B[i][q]
+    for (unsigned s = 0; s < 8; s++)
+        for (unsigned q = 0; q < 8; q++) {
+            double m = 0;
+            for (unsigned i = 0; i < 8; i++)
+                for (unsigned j = 0; j < 8; j++) m += c[i * 8 + j] * tb.biorF[j * 8 + s] * tb.biorF[i * 8 + q];
+            y[s * 8 + q] = m;
+        }
+}
+
+void bior_inv_T(const double* r, double* y, const TabT<double>& tb) {
+    // inverse: pix = S c^T S^T (S = synthesis)  =>  c'[t][s] = sum r[j][i] S[j][s] S[i][t]
+    for (unsigned t = 0; t < 8; t++)
+        for (unsigned s = 0; s < 8; s++) {
+            double m = 0;
+            for (unsigned j = 0; j < 8; j++)
+                for (unsigned i = 0; i < 8; i++) m += r[j * 8 + i] * tb.biorI[j * 8 + s] * tb.biorI[i * 8 + t];
+            y[t * 8 + s] = m;
+        }
+}
+
+// Adds the transpose of one crop's step-2 contribution, applied to u, to gz.
+void adjoint_crop(StepIn& in, const std::vector<unsigned>& rows, const std::vector<unsigned>& cols,
+                  const double* u, double* gz) {
+    const Geo& g = in.g;
+    const orc_params& pr = in.pr;
+    const unsigned k = pr.k, k2 = k * k, C = g.C, N = pr.N;
+    const bool had = pr.T_3D == kHADAMARD, bior = pr.T_2D == kBIOR;
+    Tables tables(k);
+    const TabT<double> tb = make_tab_d(tables);
+    BM bm(in.basic, g, in.ff, in.bf, pr);
+    std::vector<float> dist(N);
+    std::vector<unsigned> idx(N);
+    std::vector<double> patch(k2), coef(k2), e((size_t)C * k2 * N), gain((size_t)C * k2 * N),
+        r((size_t)C * k2 * N), tmp(k2);
+    for (unsigned t = 0; t + pr.kt <= g.F; ++t)
+        for (unsigned yr : rows)
+            for (unsigned xr : cols) {
+                const unsigned nsx = bm.run(g.index(xr, yr, t, 0), dist.data(), idx.data(), nullptr);
+                // Wiener gains and weights from the basic estimate (frozen)
+                for (unsigned c = 0; c < C; c++)
+                    for (unsigned n = 0; n < nsx; n++) {
+                        unsigned px, py, pt;
+                        g.coords(idx[n], px, py, pt);
+                        for (unsigned p = 0; p < k; p++)
+                            for (unsigned q = 0; q < k; q++)
+                                patch[p * k + q] = in.basic[g.index(px + q, py + p, pt, c)];
+                        if (bior) bior_fwd(patch.data(), coef.data(), tb);
+                        else dct_fwd(patch.data(), coef.data(), tb);
+                        for (unsigned kk = 0; kk < k2; kk++) e[n + kk * nsx + c * k2 * nsx] = coef[kk];
+                    }
+                double w[3];
+                for (unsigned c = 0; c < C; c++) {
+                    const double s2 = (double)in.sig[c] * (double)in.sig[c];
+                    const double cf = 1.0 / (double)nsx;
+                    double W2 = 0;
+                    for (unsigned kk = 0; kk < k2; kk++) {
+                        double* ev = e.data() + kk * nsx + c * k2 * nsx;
+                        if (had) hadamard(ev, nsx);
+                        else haar_fwd(ev, nsx);
+                        for (unsigned n = 0; n < nsx; n++) {
+                            double v = ev[n] * ev[n];
+                            if (had) v *= cf;
+                            v /= (v + s2);
+                            W2 += v * v;
+                            gain[n + kk * nsx + c * k2 * nsx] = had ? v * cf : v;
+                        }
+                    }
+                    w[c] = had ? (W2 > 0 ? 1.0 / (s2 * W2) : 1.0) : 1.0 / (s2 * W2);
+                }
+                // Agg^T then G^T, into group layout [c][kk][n]
+                for (unsigned c = 0; c < C; c++)
+                    for (unsigned n = 0; n < nsx; n++) {
+                        unsigned px, py, pt;
+                        g.coords(idx[n], px, py, pt);
+                        for (unsigned p = 0; p < k; p++)
+                            for (unsigned q = 0; q < k; q++)
+                                patch[p * k + q] = tb.kaiser[p * k + q] * w[c] * u[g.index(px + q, py + p, pt, c)];
+                        if (bior) bior_inv_T(patch.data(), coef.data(), tb);
+                        else dct_inv_T(patch.data(), coef.data(), tb);
+                        for (unsigned kk = 0; kk < k2; kk++) r[n + kk * nsx + c * k2 * nsx] = coef[kk];
+                    }
+                // (T3^-1 V T3)^T = T3^-T V T3^T: Haar forward, gains, Haar inverse
+                // (orthonormal); Hadamard is symmetric
+                for (unsigned a = 0; a < k2 * C; a++) {
+                    double* rv = r.data() + a * nsx;
+                    if (had) hadamard(rv, nsx);
+                    else haar_fwd(rv, nsx);
+                    for (unsigned n = 0; n < nsx; n++) rv[n] *= gain[a * nsx + n];
+                    if (had) hadamard(rv, nsx);
+                    else haar_inv(rv, nsx);
+                }
+                // F^T and the scatter back to the member patches
+                for (unsigned c = 0; c < C; c++)
+                    for (unsigned n = 0; n < nsx; n++) {
+                        unsigned px, py, pt;
+                        g.coords(idx[n], px, py, pt);
+                        for (unsigned kk = 0; kk < k2; kk++) coef[kk] = r[n + kk * nsx + c * k2 * nsx];
+                        if (bior) bior_fwd_T(coef.data(), tmp.data(), tb);
+                        else dct_fwd_T(coef.data(), tmp.data(), tb);
+                        for (unsigned p = 0; p < k; p++)
+                            for (unsigned q = 0; q < k; q++) gz[g.index(px + q, py + p, pt, c)] += tmp[p * k + q];
+                    }
+            }
+}
+
 bool prepare(StepIn& in, int step, float sigma, const float* noisy,
              const float* basic, const float* ff, const float* bf, unsigned W,
              unsigned H, unsigned F, unsigned C, const orc_params* prm,
@@ -1109,6 +1252,45 @@ int orc_step_fp64(int step, float sigma, const float* noisy, const float* basic,
     StepIn in32 = in;
     if (run_step<float>(in32, 1, 1, fn.data(), fd.data(), false, &keeps, &grps)) return -1;
     return run_step<double>(in, 1, 1, num, den, true, &keeps, &grps);
+}
+
+int orc_step2_adjoint_fp64(float sigma, const float* noisy, const float* basic,
+                           const float* fflow, const float* bflow, unsigned W,
+                           unsigned H, unsigned F, unsigned C, const orc_params* prm,
+                           int color_space, int nb, const double* grad_out,
+                           double* grad_noisy, double* final_out) {
+    StepIn in;
+    if (!basic) { g_err = "step 2 needs the basic estimate"; return -1; }
+    if (!prepare(in, 2, sigma, noisy, basic, fflow, bflow, W, H, F, C, prm, color_space)) return -1;
+    Grid grid;
+    if (!make_grid(grid, W, H, prm->k, prm->p, prm->n, nb)) { g_err = "invalid reference grid"; return -1; }
+    // den (and the forward value) in FP64 with the FP32 run's groups
+    std::vector<std::vector<uint8_t>> keeps;
+    std::vector<std::vector<uint32_t>> grps;
+    const size_t n = (size_t)W * H * F * C;
+    std::vector<float> fn(n), fd(n);
+    std::vector<double> num(n), den(n), u(n);
+    StepIn in32 = in;
+    if (run_step<float>(in32, nb, 1, fn.data(), fd.data(), false, &keeps, &grps)) return -1;
+    StepIn in64 = in;
+    if (run_step<double>(in64, nb, 1, num.data(), den.data(), true, &keeps, &grps)) return -1;
+    for (size_t i = 0; i < n; i++) {
+        u[i] = den[i] != 0 ? grad_out[i] / den[i] : 0.0;
+        grad_noisy[i] = 0.0;
+        if (final_out)
+            final_out[i] = den[i] != 0 ? num[i] / den[i] : (nb == 1 ? (double)basic[i] : (double)noisy[i]);
+    }
+    try {
+        for (size_t c = 0; c < grid.rows.size(); c++) adjoint_crop(in, grid.rows[c], grid.cols[c], u.data(), grad_noisy);
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return -1;
+    }
+    // den == 0 keeps the noisy value when nb > 1 (src/vbm3d.cpp:343)
+    if (nb > 1)
+        for (size_t i = 0; i < n; i++)
+            if (den[i] == 0) grad_noisy[i] += grad_out[i];
+    return 0;
 }
 
 int orc_run_vbm3d(float sigma, float* noisy, const float* fflow,

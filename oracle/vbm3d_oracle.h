@@ -88,6 +88,18 @@ int orc_step_fp64(int step, float sigma, const float* noisy, const float* basic,
                   unsigned H, unsigned F, unsigned C, const orc_params* prm,
                   int color_space, double* num, double* den);
 
+/* Adjoint of step 2 with the basic estimate fixed (so the groups, Wiener
+ * gains and weights are frozen), in FP64: grad_noisy = J^T grad_out,
+ * where J is the Jacobian of the step-2 output num/den (with the den == 0
+ * fallback) with respect to the colour-transformed noisy video.
+ * final_out (may be NULL) receives the FP64 forward value.  Runs
+ * single-threaded: meant for small clips. */
+int orc_step2_adjoint_fp64(float sigma, const float* noisy, const float* basic,
+                           const float* fflow, const float* bflow, unsigned W,
+                           unsigned H, unsigned F, unsigned C, const orc_params* prm,
+                           int color_space, int nb, const double* grad_out,
+                           double* grad_noisy, double* final_out);
+
 /* Building blocks, exposed for the oracle's own pinning tests. */
 void orc_bior_fwd(const float* patch64, float* out64);
 void orc_bior_inv(float* sig64);

@@ -6,8 +6,8 @@ libvbm3d_hip.so).  The Python API mirrors the reference's interface.
 """
 from .api import (BIOR, DCT, HAAR, HADAMARD, NONE, Context, Parameters, VBM3DError, block_match,
                   denoise, grid, initialize_parameters_1, initialize_parameters_2, lib, run_device,
-                  run_vbm3d, step)
+                  run_vbm3d, step, step2_adjoint, sure)
 
 __all__ = ["BIOR", "DCT", "HAAR", "HADAMARD", "NONE", "Context", "Parameters", "VBM3DError",
            "block_match", "denoise", "grid", "initialize_parameters_1", "initialize_parameters_2",
-           "lib", "run_device", "run_vbm3d", "step"]
+           "lib", "run_device", "run_vbm3d", "step", "step2_adjoint", "sure"]

@@ -71,6 +71,9 @@ def lib():
             L.vbm3d_run_device.argtypes = [C.c_void_p, C.c_float, C.c_void_p, C.c_void_p, C.c_void_p,
                                            C.c_uint, C.c_uint, C.c_uint, C.c_uint, pp, pp, C.c_int, C.c_int,
                                            C.c_void_p, C.c_void_p, C.c_void_p]
+            L.vbm3d_step2_adjoint.argtypes = [C.c_void_p, C.c_float, C.c_void_p, C.c_void_p, C.c_void_p,
+                                              C.c_void_p, C.c_uint, C.c_uint, C.c_uint, C.c_uint, pp, C.c_int,
+                                              C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
             L.vbm3d_block_match.argtypes = [C.c_void_p, fp, C.c_uint, C.c_uint, C.c_uint, C.c_uint, fp, fp, pp,
                                             C.c_long, up, up, fp, up, up]
             L.vbm3d_step.argtypes = [C.c_void_p, C.c_int, C.c_float, fp, fp, fp, fp, C.c_uint, C.c_uint,
@@ -207,6 +210,56 @@ def run_device(sigma, noisy_t, basic_t, final_t, prms_1, prms_2, color_space=Tru
                                   int(ref_grid_nb), C.c_void_p(basic_t.data_ptr()),
                                   C.c_void_p(final_t.data_ptr()), sp))
     return 0
+
+
+def _ptr(t):
+    return None if t is None else C.c_void_p(t.data_ptr())
+
+
+def step2_adjoint(sigma, noisy_t, basic_t, grad_t, prms_2, color_space=True, ref_grid_nb=1,
+                  fflow_t=None, bflow_t=None, final_t=None, stream=None, ctx: Context | None = None):
+    """J^T grad_t for step 2 with the basic estimate fixed (groups, Wiener
+    gains and weights frozen), on torch CUDA tensors [F][C][H][W] in the
+    colour-transformed space; the step-2 output goes to final_t if given.
+    Returns the gradient with respect to noisy_t (a new tensor)."""
+    F, Cc, H, W = noisy_t.shape
+    for t in (noisy_t, basic_t, grad_t):
+        if tuple(t.shape) != (F, Cc, H, W) or not t.is_contiguous():
+            raise ValueError("noisy, basic and grad must be contiguous and of one shape")
+    ctx = ctx or default_context()
+    out = noisy_t.new_empty(noisy_t.shape)
+    sp = None if stream is None else C.c_void_p(int(stream.cuda_stream))
+    _check(lib().vbm3d_step2_adjoint(ctx.h, float(sigma), _ptr(noisy_t), _ptr(basic_t), _ptr(fflow_t),
+                                     _ptr(bflow_t), W, H, F, Cc, C.byref(prms_2), int(bool(color_space)),
+                                     int(ref_grid_nb), _ptr(grad_t), _ptr(out), _ptr(final_t), sp))
+    return out
+
+
+def sure(sigma_noise, noisy_t, prms_1, prms_2, sigma=None, probe_seed=0, ref_grid_nb=1,
+         stream=None, ctx: Context | None = None):
+    """Monte-Carlo SURE of the denoiser on a grey clip without clean frames:
+        SURE = |f(y) - y|^2 / n - s^2 + 2 s^2 div f / n,   s = sigma_noise,
+    with div f estimated by one Rademacher probe b as b . J^T b, J the
+    step-2 Jacobian with the basic estimate frozen (step 1's dependence on
+    y is not differentiated).  `sigma` is the denoiser's parameter
+    (default sigma_noise).  Returns (sure, divergence, final)."""
+    import torch
+    if noisy_t.shape[1] != 1:
+        raise ValueError("sure() takes grey clips (the divergence is taken in the filter's space)")
+    sigma = float(sigma_noise if sigma is None else sigma)
+    y = noisy_t.contiguous()
+    work = y.clone()
+    basic = torch.empty_like(y)
+    final = torch.empty_like(y)
+    run_device(sigma, work, basic, final, prms_1, prms_2, False, ref_grid_nb, stream=stream, ctx=ctx)
+    g = torch.Generator(device="cpu").manual_seed(int(probe_seed))
+    b = (torch.randint(0, 2, tuple(y.shape), generator=g, dtype=torch.int8).to(torch.float32) * 2 - 1).to(y.device)
+    jtb = step2_adjoint(sigma, y, basic, b, prms_2, False, ref_grid_nb, stream=stream, ctx=ctx)
+    n = y.numel()
+    div = float((b.double() * jtb.double()).sum())
+    s2 = float(sigma_noise) ** 2
+    res = float(((final.double() - y.double()) ** 2).sum()) / n
+    return res - s2 + 2.0 * s2 * div / n, div, final
 
 
 def block_match(vid, prm, ref_idx, fflow=None, bflow=None, ctx=None):

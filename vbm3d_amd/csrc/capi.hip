@@ -278,6 +278,9 @@ struct StepIO {
     float* num_raw;        // may be null
     float* den_raw;
     unsigned maxflow;
+    bool adjoint;          // step-2 adjoint: num_raw receives J^T gy
+    const float* gy;
+    const float* den;      // the forward pass's denominator
 };
 
 // Brackets one launch with events when profiling is on.
@@ -380,6 +383,8 @@ int run_step(vbm3d_ctx* ctx, const StepIO& io, hipStream_t s) {
     fa.tabs = (const Tabs*)ctx->tabs.p;
     fa.est = (float*)ctx->est.p;
     fa.wgt = (float*)ctx->wgt.p;
+    fa.gy = io.gy;
+    fa.den = io.den;
 
     AArgs ag{};
     ag.W = io.W; ag.H = io.H; ag.F = io.F; ag.C = C; ag.wh = bm.wh; ag.whc = bm.whc;
@@ -396,6 +401,7 @@ int run_step(vbm3d_ctx* ctx, const StepIO& io, hipStream_t s) {
     ag.out = io.out;
     ag.num_out = io.num_raw;
     ag.den_out = io.den_raw;
+    ag.raw = io.adjoint ? 1 : 0;
 
     unsigned t_done = 0;
     for (unsigned a = 0; a < Fr; a += S) {
@@ -406,7 +412,10 @@ int run_step(vbm3d_ctx* ctx, const StepIO& io, hipStream_t s) {
         if (ctx->count)
             HIP_TRY(launch_sum_counts(bm.count_out, bm.r_end - bm.r_begin,
                                       (unsigned long long*)ctx->stats.p + 4 * (io.step - 1), s));
-        HIP_TRY(timed(ctx, 1, io.step, s, [&] { return launch_filter((int)K, io.step, fa, s); }));
+        if (io.adjoint)
+            HIP_TRY(launch_filter_adjoint((int)K, fa, s));
+        else
+            HIP_TRY(timed(ctx, 1, io.step, s, [&] { return launch_filter((int)K, io.step, fa, s); }));
         const unsigned ready = e >= Fr ? io.F : (e > p.Nf ? e - p.Nf : 0);
         if (ready > t_done) {
             ag.t_begin = t_done;
@@ -640,6 +649,51 @@ int vbm3d_step(vbm3d_ctx* ctx, int step, float sigma, const float* noisy, const 
     HIP_TRY(hipStreamSynchronize(s));
     HIP_TRY(hipMemcpy(num, ctx->num_raw.p, n * 4, hipMemcpyDeviceToHost));
     HIP_TRY(hipMemcpy(den, ctx->den_raw.p, n * 4, hipMemcpyDeviceToHost));
+    return VBM3D_OK;
+}
+
+int vbm3d_step2_adjoint(vbm3d_ctx* ctx, float sigma, const float* d_noisy, const float* d_basic,
+                        const float* d_fflow, const float* d_bflow, unsigned W, unsigned H,
+                        unsigned F, unsigned C, const vbm3d_params* p2, int color_space,
+                        int ref_grid_nb, const float* d_grad_out, float* d_grad_noisy,
+                        float* d_final, void* stream) {
+    if (!ctx || !d_noisy || !d_basic || !p2 || !d_grad_out || !d_grad_noisy)
+        return fail(VBM3D_ERR_ARG, "null argument");
+    HIP_TRY(hipSetDevice(ctx->device));
+    hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
+    int rc = check(*p2, W, H, F, C, 2);
+    if (rc) return rc;
+    const size_t n = (size_t)W * H * F * C;
+    HIP_TRY(ensure(ctx->num_raw, n * 4));
+    HIP_TRY(ensure(ctx->den_raw, n * 4));
+    unsigned maxflow = 0;
+    rc = flow_absmax(ctx, d_fflow, d_bflow, W, H, F, s, &maxflow);
+    if (rc) return rc;
+    StepIO io{};
+    io.step = 2; io.sigma = sigma;
+    io.noisy = d_noisy; io.basic = d_basic; io.ff = d_fflow; io.bf = d_bflow;
+    io.W = W; io.H = H; io.F = F; io.C = C; io.p = p2; io.cs = color_space != 0; io.nb = ref_grid_nb;
+    io.maxflow = maxflow;
+    // forward pass: the denominator (and, if asked, the output)
+    io.out = d_final;
+    io.fallback = ref_grid_nb == 1 ? d_basic : d_noisy;
+    io.num_raw = (float*)ctx->num_raw.p;
+    io.den_raw = (float*)ctx->den_raw.p;
+    rc = run_step(ctx, io, s);
+    if (rc) return rc;
+    // transposed pass: the same groups (recomputed), gathered gradient
+    io.adjoint = true;
+    io.gy = d_grad_out;
+    io.den = (const float*)ctx->den_raw.p;
+    io.out = nullptr;
+    io.fallback = nullptr;
+    io.num_raw = d_grad_noisy;
+    io.den_raw = nullptr;
+    rc = run_step(ctx, io, s);
+    if (rc) return rc;
+    // den == 0 keeps the noisy value when the grid is tiled (src/vbm3d.cpp:343)
+    if (ref_grid_nb > 1) HIP_TRY(launch_adjoint_fallback(d_grad_noisy, d_grad_out, io.den, n, s));
+    HIP_TRY(hipStreamSynchronize(s));
     return VBM3D_OK;
 }
 

@@ -96,6 +96,8 @@ struct FArgs {
     const Tabs* tabs;
     float* est;  // [slot][C][N][K*K]
     float* wgt;  // [slot][C]
+    const float* gy;   // step-2 adjoint: gradient with respect to the output
+    const float* den;  // step-2 adjoint: the forward pass's denominator
 };
 
 // Exact unsigned 32-bit division by a run-time constant d >= 1 with a
@@ -137,7 +139,8 @@ struct AArgs {
     const float* fallback;      // used where den == 0
     float* out;                 // out = den == 0 ? fallback : num / den
     float* num_out;             // optional raw sums
-    float* den_out;
+    float* den_out;             // optional
+    int raw;                    // 1: plain sums of est (no Kaiser window, no weight)
 };
 
 // Launchers (kernels.hip).  Return hipSuccess or the launch error.
@@ -146,6 +149,8 @@ hipError_t launch_filter(int K, int step, const FArgs& a, hipStream_t s);
 hipError_t launch_aggregate(int C, const AArgs& a, unsigned frames, hipStream_t s);
 hipError_t launch_color(float* v, unsigned W, unsigned H, unsigned F, bool forward,
                         hipStream_t s);
+hipError_t launch_filter_adjoint(int K, const FArgs& a, hipStream_t s);
+hipError_t launch_adjoint_fallback(float* gz, const float* gy, const float* den, size_t n, hipStream_t s);
 hipError_t launch_sum_counts(const uint32_t* counts, long n, unsigned long long* stats, hipStream_t s);
 hipError_t launch_flow_absmax(const float* fl, size_t n, unsigned* out, hipStream_t s);
 

@@ -699,6 +699,227 @@ __global__ __launch_bounds__(256) void k_filter(FArgs a) {
 }
 
 // ---------------------------------------------------------------------
+// Adjoint of step 2 with the groups, Wiener gains and weights frozen by
+// the basic estimate.  Step 2 is then linear in the noisy video:
+//   final = D^-1 sum_g Agg_g (kaiser w_g) G T3^-1 V_g T3 F Gather_g z,
+// so its transpose gathers u = grad / den, applies the transposes of the
+// inverse 2D transform G, of the group operator (T3^-1 V T3)^T, which is
+// T3^-1 V T3 again for the orthonormal Haar and for the symmetric
+// Hadamard, and of the forward 2D transform F, scaled by w_g.  The
+// per-member results go to `est`; the aggregation kernel in raw mode then
+// sums them over every member patch covering a pixel.  Lane = coefficient
+// (or pixel) index; these are plain FP32 evaluations of the transposed
+// operators, not bit-replays of a reference computation.
+// ---------------------------------------------------------------------
+// Forward 2D transform of the K x K patch `in` (LDS, row-major) as the
+// forward pass computes it; returns coefficient `lane`.
+template <int K>
+__device__ float adj_fwd2d(const float* in, float* tmp, int lane, bool bior, const Tabs& T) {
+    constexpr int K2 = K * K;
+    const bool act = lane < K2;
+    const int a = lane / K, b = lane - (lane / K) * K;
+    float o = 0.f;
+    if (bior) {
+        if (act) {
+            float m = 0.f;
+            for (int s = 0; s < 8; s++) m = m + in[a * 8 + s] * T.biorF[b * 8 + s];
+            tmp[lane] = m;
+        }
+        wave_sync();
+        if (act)
+            for (int s = 0; s < 8; s++) o = o + tmp[s * 8 + a] * T.biorF[b * 8 + s];
+    } else {
+        if (act) {
+            float m = 0.f;
+            for (int q = 0; q < K; q++) m = m + in[a * K + q] * T.dctF[b * K + q];
+            tmp[lane] = m;
+        }
+        wave_sync();
+        if (act) {
+            for (int p = 0; p < K; p++) o = o + tmp[p * K + b] * T.dctF[a * K + p];
+            o = o * T.nrm[lane];
+        }
+    }
+    wave_sync();
+    return o;
+}
+
+// Transpose of the inverse 2D transform applied to the pixel tile `in`;
+// returns coefficient `lane`.  DCT: G(c) = dct_out M (c o inv) M^T with
+// M[x][y] = dctI[x*K+y], so G^T(r) = inv o dct_out (M^T r M).  BIOR:
+// G(c)[u][m] = sum S[u][p] c[q][p] S[m][q] with S = biorI, so
+// G^T(r)[q][p] = sum r[u][m] S[u][p] S[m][q].
+template <int K>
+__device__ float adj_inv2d_T(const float* in, float* tmp, int lane, bool bior, const Tabs& T) {
+    constexpr int K2 = K * K;
+    const bool act = lane < K2;
+    const int a = lane / K, b = lane - (lane / K) * K;
+    float o = 0.f;
+    if (bior) {
+        if (act) {  // t[u][q] = sum_m r[u][m] S[m][q]
+            float m = 0.f;
+            for (int mm = 0; mm < 8; mm++) m = m + in[a * 8 + mm] * T.biorI[mm * 8 + b];
+            tmp[lane] = m;
+        }
+        wave_sync();
+        if (act)  // c'[q = a][p = b] = sum_u S[u][p] t[u][q]
+            for (int u = 0; u < 8; u++) o = o + T.biorI[u * 8 + b] * tmp[u * 8 + a];
+    } else {
+        if (act) {  // t[u][b] = sum_m r[u][m] M[m][b]
+            float m = 0.f;
+            for (int mm = 0; mm < K; mm++) m = m + in[a * K + mm] * T.dctI[mm * K + b];
+            tmp[lane] = m;
+        }
+        wave_sync();
+        if (act) {  // c'[a][b] = inv dct_out sum_u M[u][a] t[u][b]
+            for (int u = 0; u < K; u++) o = o + T.dctI[u * K + a] * tmp[u * K + b];
+            o = T.inv[lane] * T.dct_out * o;
+        }
+    }
+    wave_sync();
+    return o;
+}
+
+// Transpose of the forward 2D transform applied to the coefficient tile
+// `in`; returns pixel `lane`.  DCT: F(x) = nrm o (A x A^T), A = dctF, so
+// F^T(c) = A^T (nrm o c) A.  BIOR: F(x) = B x^T B^T, B = biorF, so
+// F^T(c)[s][q] = sum c[i][j] B[j][s] B[i][q].
+template <int K>
+__device__ float adj_fwd2d_T(const float* in, float* tmp, int lane, bool bior, const Tabs& T) {
+    constexpr int K2 = K * K;
+    const bool act = lane < K2;
+    const int a = lane / K, b = lane - (lane / K) * K;
+    float o = 0.f;
+    if (bior) {
+        if (act) {  // t[i][s] = sum_j c[i][j] B[j][s]
+            float m = 0.f;
+            for (int j = 0; j < 8; j++) m = m + in[a * 8 + j] * T.biorF[j * 8 + b];
+            tmp[lane] = m;
+        }
+        wave_sync();
+        if (act)  // y[s = a][q = b] = sum_i t[i][s] B[i][q]
+            for (int i = 0; i < 8; i++) o = o + tmp[i * 8 + a] * T.biorF[i * 8 + b];
+    } else {
+        if (act) {  // t[u][q] = sum_v nrm[u][v] c[u][v] A[v][q]
+            float m = 0.f;
+            for (int v = 0; v < K; v++) m = m + T.nrm[a * K + v] * in[a * K + v] * T.dctF[v * K + b];
+            tmp[lane] = m;
+        }
+        wave_sync();
+        if (act)  // y[p = a][q = b] = sum_u A[u][p] t[u][q]
+            for (int u = 0; u < K; u++) o = o + T.dctF[u * K + a] * tmp[u * K + b];
+    }
+    wave_sync();
+    return o;
+}
+
+// Group operator of one channel, transposed: Wiener gains from the basic
+// coefficients eg, then T3^-1 V T3 applied to zg in place; v^2 to v2 at
+// the flat index n + kk*nSx.
+template <int K, int NS>
+__device__ void adj_group(float* zg, const float* eg, float* v2, int lane, bool had, float sig) {
+    constexpr int K2 = K * K;
+    const bool act = lane < K2;
+    float z[NS], e[NS];
+#pragma unroll
+    for (int n = 0; n < NS; n++) {
+        z[n] = act ? zg[n * K2 + lane] : 0.f;
+        e[n] = act ? eg[n * K2 + lane] : 0.f;
+    }
+    t3_fwd<NS>(e, had);
+    t3_fwd<NS>(z, had);
+    const float s2 = sig * sig;
+    const float coef = 1.0f / (float)NS;
+#pragma unroll
+    for (int n = 0; n < NS; n++) {
+        float value = e[n] * e[n];
+        if (had) value = value * coef;
+        value /= (value + s2);
+        z[n] = had ? z[n] * value * coef : z[n] * value;
+        if (act) v2[lane * NS + n] = value * value;
+    }
+    t3_inv<NS>(z, had);
+#pragma unroll
+    for (int n = 0; n < NS; n++)
+        if (act) zg[n * K2 + lane] = z[n];
+}
+
+__host__ __device__ inline int adjoint_lds_words(int K, int N) { return 2 * K * K + 3 * N * K * K; }
+
+template <int K, int C>
+__global__ __launch_bounds__(256) void k_filter_adjoint(FArgs a) {
+    constexpr int K2 = K * K;
+    extern __shared__ float smem[];
+    const int lane = threadIdx.x & 63;
+    const int w = threadIdx.x >> 6;
+    const int nw = blockDim.x >> 6;
+    const long r = a.r_begin + (long)blockIdx.x * nw + w;
+    if (r >= a.r_end) return;
+    float* P = smem + (size_t)w * adjoint_lds_words(K, a.N);  // [K2] tile
+    float* Tm = P + K2;                                         // [K2] scratch
+    float* zg = Tm + K2;                                        // [N][K2]
+    float* eg = zg + a.N * K2;                                  // [N][K2]
+    float* v2 = eg + a.N * K2;                                  // [K2*N]
+    const unsigned t = (unsigned)(r / a.nrf), i = (unsigned)(r - (long)t * a.nrf);
+    const unsigned slot = __builtin_amdgcn_readfirstlane((t % a.ring) * a.nrf + i);
+    const unsigned W = a.W, wh = a.wh, whc = a.whc;
+    const int nsx = (int)__builtin_amdgcn_readfirstlane(a.nsx[slot]);
+    const bool bior = a.T2D == 1, had = a.T3D == 2;
+    const Tabs& T = *a.tabs;
+    const uint32_t* mem = a.idx + (size_t)slot * a.N;
+    const bool act = lane < K2;
+    const int pr = lane / K, pc = lane - (lane / K) * K;
+
+    for (int c = 0; c < C; c++) {
+        for (int n = 0; n < nsx; n++) {
+            const unsigned mi = mem[n];
+            const unsigned px = mi % W, py = (mi % wh) / W, pt = mi / whc;
+            const size_t o = (size_t)pt * whc + (size_t)c * wh + (size_t)(py + pr) * W + px + pc;
+            if (act) P[lane] = a.basic[o];
+            wave_sync();
+            const float ev = adj_fwd2d<K>(P, Tm, lane, bior, T);
+            if (act) {
+                eg[n * K2 + lane] = ev;
+                const float dn = a.den[o];
+                P[lane] = T.kaiser[lane] * (dn != 0.f ? a.gy[o] / dn : 0.f);
+            }
+            wave_sync();
+            const float zv = adj_inv2d_T<K>(P, Tm, lane, bior, T);
+            if (act) zg[n * K2 + lane] = zv;
+        }
+        wave_sync();
+        switch (nsx) {
+            case 1: adj_group<K, 1>(zg, eg, v2, lane, had, a.sig[c]); break;
+            case 2: adj_group<K, 2>(zg, eg, v2, lane, had, a.sig[c]); break;
+            case 4: adj_group<K, 4>(zg, eg, v2, lane, had, a.sig[c]); break;
+            case 8: adj_group<K, 8>(zg, eg, v2, lane, had, a.sig[c]); break;
+            default: adj_group<K, 16>(zg, eg, v2, lane, had, a.sig[c]); break;
+        }
+        wave_sync();
+        // the group's aggregation weight, summed in the forward pass's order
+        float acc = 0.f;
+        if (lane == 0)
+            for (int f = 0; f < K2 * nsx; f++) acc = acc + v2[f];
+        acc = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(acc), 0));
+        const float s2 = a.sig[c] * a.sig[c];
+        const float wg = had ? (acc > 0.0f ? 1.0f / (s2 * acc) : 1.0f) : 1.f / (s2 * acc);
+        float* dst = a.est + (((size_t)slot * C + c) * a.N) * K2;
+        for (int n = 0; n < nsx; n++) {
+            if (act) P[lane] = zg[n * K2 + lane];
+            wave_sync();
+            const float y = adj_fwd2d_T<K>(P, Tm, lane, bior, T);
+            if (act) dst[(size_t)n * K2 + lane] = wg * y;
+        }
+        wave_sync();
+    }
+}
+
+__global__ void k_adjoint_fallback(float* gz, const float* gy, const float* den, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && den[i] == 0.f) gz[i] = gz[i] + gy[i];
+}
+
+// ---------------------------------------------------------------------
 // Aggregation (src/vbm3d.cpp:612-650, 860-924) with the division and the
 // den == 0 fallback (:168-170, :198-202, :267-280, :333-346).
 // ---------------------------------------------------------------------
@@ -823,8 +1044,9 @@ __global__ __launch_bounds__(256) void k_aggregate(AArgs a) {
                         const float kai = s_kai[kk];
 #pragma unroll
                         for (int c = 0; c < C; c++) {
-                            const float kw = kai * a.wgt[(size_t)sl * C + c];
-                            pn[c] = pn[c] + kw * a.est[(((size_t)sl * C + c) * N + n) * K2 + kk];
+                            const float kw = a.raw ? 1.f : kai * a.wgt[(size_t)sl * C + c];
+                            const float ev = a.est[(((size_t)sl * C + c) * N + n) * K2 + kk];
+                            pn[c] = pn[c] + (a.raw ? ev : kw * ev);
                             pd[c] = pd[c] + kw;
                         }
                     }
@@ -847,10 +1069,8 @@ __global__ __launch_bounds__(256) void k_aggregate(AArgs a) {
 #pragma unroll
         for (int c = 0; c < C; c++) {
             const size_t o = (size_t)to * whc + (size_t)c * wh + (size_t)py * W + px;
-            if (a.num_out) {
-                a.num_out[o] = tn[c];
-                a.den_out[o] = td[c];
-            }
+            if (a.num_out) a.num_out[o] = tn[c];
+            if (a.den_out) a.den_out[o] = td[c];
             if (a.out) a.out[o] = td[c] == 0.f ? a.fallback[o] : tn[c] / td[c];
         }
     }
@@ -973,6 +1193,37 @@ hipError_t launch_filter(int K, int step, const FArgs& a, hipStream_t s) {
         case 8: return filter_kc<8>(step, a, s);
         default: return hipErrorInvalidValue;
     }
+}
+
+template <int K>
+static hipError_t filter_adj_k(const FArgs& a, hipStream_t s) {
+    const long n = a.r_end - a.r_begin;
+    if (n <= 0) return hipSuccess;
+    const size_t per_wave = (size_t)adjoint_lds_words(K, a.N) * 4;
+    int waves = 4;
+    while (waves > 1 && per_wave * waves > 64 * 1024) waves--;
+    if (per_wave > 64 * 1024) return hipErrorInvalidValue;
+    const dim3 grid((unsigned)((n + waves - 1) / waves)), block(64 * waves);
+    if (a.C == 1) hipLaunchKernelGGL((k_filter_adjoint<K, 1>), grid, block, per_wave * waves, s, a);
+    else hipLaunchKernelGGL((k_filter_adjoint<K, 3>), grid, block, per_wave * waves, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_filter_adjoint(int K, const FArgs& a, hipStream_t s) {
+    switch (K) {
+        case 4: return filter_adj_k<4>(a, s);
+        case 5: return filter_adj_k<5>(a, s);
+        case 6: return filter_adj_k<6>(a, s);
+        case 7: return filter_adj_k<7>(a, s);
+        case 8: return filter_adj_k<8>(a, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_adjoint_fallback(float* gz, const float* gy, const float* den, size_t n, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_adjoint_fallback, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, gz, gy, den, n);
+    return hipGetLastError();
 }
 
 hipError_t launch_aggregate(int C, const AArgs& a, unsigned frames, hipStream_t s) {
