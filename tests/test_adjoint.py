@@ -136,10 +136,14 @@ def cfg5(torch_dev):
 @pytest.mark.gpu
 def test_gpu_adjoint_dot_product_and_autograd_fd(gpu_ctx, torch_dev, cfg5):
     """On the 640x360x32 clip: <J x, y> = <x, J^T y>, then the autograd
-    gradient of L = <g, f(z)> against central differences at 256 random
-    pixels.  f is affine, so the step only trades FP32 output rounding:
-    eps = 1e-3 is checked to the rounding bound (the output's ulp over
-    2 eps, summed over the outputs a pixel reaches) and eps = 1 tightly."""
+    gradient of L = <y, f(z)> against central differences at 256 random
+    pixels.  With the basic estimate fixed, f is affine, so the difference
+    quotient is the same at every base point; only FP32 output rounding
+    differs.  Around the noisy clip (outputs near 100, ulp 7.6e-6) the
+    rounding of the thousands of outputs a pixel reaches limits the
+    quotient to about 1e-3 relative even at eps = 1; around z = 0 the
+    outputs are of the size of the step, so eps = 1e-3 resolves the
+    gradient to FP32 accuracy."""
     import vbm3d_amd as V
     torch, dev = torch_dev
     sigma, _, noisy, z, basic, final = cfg5
@@ -163,10 +167,10 @@ def test_gpu_adjoint_dot_product_and_autograd_fd(gpu_ctx, torch_dev, cfg5):
     idx = rng.choice(z.numel(), 256, replace=False)
     flat = z.reshape(-1)
 
-    def fd(eps):
+    def fd(eps, base):
         out = np.empty(256)
         for j, i in enumerate(idx):
-            zp, zm = flat.clone(), flat.clone()
+            zp, zm = base.clone(), base.clone()
             zp[i] += eps
             zm[i] -= eps
             h = float(zp[i].double() - zm[i].double())
@@ -176,13 +180,14 @@ def test_gpu_adjoint_dot_product_and_autograd_fd(gpu_ctx, torch_dev, cfg5):
         return out
 
     g_adj = grad.reshape(-1)[torch.from_numpy(idx).to(dev)].cpu().numpy()
-    g1 = fd(1.0)
-    rel1 = np.linalg.norm(g_adj - g1) / np.linalg.norm(g1)
-    g3 = fd(1e-3)
+    g3 = fd(1e-3, torch.zeros_like(flat))
     rel3 = np.linalg.norm(g_adj - g3) / np.linalg.norm(g3)
-    print(f"FD eps=1: rel err {rel1:.3g}; eps=1e-3: rel err {rel3:.3g}; |grad| {np.linalg.norm(g1):.4g}")
-    assert rel1 < 1e-3
-    assert rel3 < 0.15
+    g1 = fd(1.0, flat)
+    rel1 = np.linalg.norm(g_adj - g1) / np.linalg.norm(g1)
+    print(f"FD eps=1e-3 at 0: rel err {rel3:.3g} (max abs {np.abs(g_adj - g3).max():.3g}); "
+          f"eps=1 at the noisy clip: rel err {rel1:.3g}; |grad| {np.linalg.norm(g3):.4g}")
+    assert rel3 < 1e-4
+    assert rel1 < 1e-2
 
 
 @pytest.mark.gpu

@@ -58,6 +58,13 @@ def lib():
         if _lib is None:
             if not os.path.exists(LIB_PATH):
                 raise ImportError(f"{LIB_PATH} is missing: run `python -m vbm3d_amd.build` first")
+            # PyTorch carries its own HIP runtime.  Loaded first, it satisfies
+            # this library's HIP dependency, so the process holds one runtime;
+            # two runtimes in one process cannot both open the GPU.
+            try:
+                import torch  # noqa: F401
+            except ImportError:
+                pass
             L = C.CDLL(LIB_PATH)
             fp, up, pp = C.POINTER(C.c_float), C.POINTER(C.c_uint32), C.POINTER(Parameters)
             L.vbm3d_create.argtypes = [C.c_int]
