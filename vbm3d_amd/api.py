@@ -242,28 +242,55 @@ def step2_adjoint(sigma, noisy_t, basic_t, grad_t, prms_2, color_space=True, ref
     return out
 
 
-def sure(sigma_noise, noisy_t, prms_1, prms_2, sigma=None, probe_seed=0, ref_grid_nb=1,
-         stream=None, ctx: Context | None = None):
+def sure(sigma_noise, noisy_t, prms_1, prms_2, sigma=None, basic_t=None, probe_seed=0, ref_grid_nb=1,
+         pilot_eps=0.01, stream=None, ctx: Context | None = None):
     """Monte-Carlo SURE of the denoiser on a grey clip without clean frames:
         SURE = |f(y) - y|^2 / n - s^2 + 2 s^2 div f / n,   s = sigma_noise,
     with div f estimated by one Rademacher probe b as b . J^T b, J the
-    step-2 Jacobian with the basic estimate frozen (step 1's dependence on
-    y is not differentiated).  `sigma` is the denoiser's parameter
-    (default sigma_noise).  Returns (sure, divergence, final)."""
+    step-2 Jacobian with the basic estimate frozen (the hand-written
+    adjoint).  When the pilot is computed here from y, its own share of
+    the divergence -- y -> step 1 -> basic -> Wiener gains -> output -- is
+    added as a central difference along b with step pilot_eps*sigma_noise
+    (0 leaves it out).  `sigma` is the denoiser's parameter (default
+    sigma_noise).  With basic_t given, step 1 is skipped and that pilot is
+    used as a constant.  Returns (sure, divergence, final)."""
     import torch
     if noisy_t.shape[1] != 1:
         raise ValueError("sure() takes grey clips (the divergence is taken in the filter's space)")
     sigma = float(sigma_noise if sigma is None else sigma)
     y = noisy_t.contiguous()
     work = y.clone()
-    basic = torch.empty_like(y)
     final = torch.empty_like(y)
-    run_device(sigma, work, basic, final, prms_1, prms_2, False, ref_grid_nb, stream=stream, ctx=ctx)
+    if basic_t is None:
+        basic = torch.empty_like(y)
+        p1 = prms_1
+    else:
+        basic = basic_t.contiguous().clone()
+        p1 = prms_1.copy()
+        p1.k = 0
+    run_device(sigma, work, basic, final, p1, prms_2, False, ref_grid_nb, stream=stream, ctx=ctx)
     g = torch.Generator(device="cpu").manual_seed(int(probe_seed))
     b = (torch.randint(0, 2, tuple(y.shape), generator=g, dtype=torch.int8).to(torch.float32) * 2 - 1).to(y.device)
     jtb = step2_adjoint(sigma, y, basic, b, prms_2, False, ref_grid_nb, stream=stream, ctx=ctx)
     n = y.numel()
     div = float((b.double() * jtb.double()).sum())
+    if basic_t is None and pilot_eps:
+        # the pilot's share of the divergence, b . (d final / d basic)(d basic / d y) b,
+        # as a central difference through step 1 and the Wiener gains
+        skip2 = prms_2.copy()
+        skip2.k = 0
+        skip1 = prms_1.copy()
+        skip1.k = 0
+        eps = float(pilot_eps) * float(sigma_noise)
+        outs = []
+        for sgn in (1.0, -1.0):
+            yp = (y + sgn * eps * b).contiguous()
+            bp, fp = torch.empty_like(y), torch.empty_like(y)
+            run_device(sigma, yp.clone(), bp, fp, prms_1, skip2, False, ref_grid_nb, stream=stream, ctx=ctx)
+            f2 = torch.empty_like(y)
+            run_device(sigma, y.clone(), bp, f2, skip1, prms_2, False, ref_grid_nb, stream=stream, ctx=ctx)
+            outs.append(f2)
+        div += float((b.double() * (outs[0].double() - outs[1].double())).sum()) / (2.0 * eps)
     s2 = float(sigma_noise) ** 2
     res = float(((final.double() - y.double()) ** 2).sum()) / n
     return res - s2 + 2.0 * s2 * div / n, div, final
