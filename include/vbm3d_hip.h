@@ -137,6 +137,55 @@ int vbm3d_step2_adjoint(vbm3d_ctx* ctx, float sigma, const float* d_noisy,
                         const float* d_grad_out, float* d_grad_noisy,
                         float* d_final, void* stream);
 
+/* One clip of a batch (no counterpart in the reference, which denoises
+ * one clip per call).  Device buffers, planar [F][C][H][W]; the same
+ * semantics per clip as vbm3d_run_device: `noisy` is mutated by the
+ * colour round trip, `basic` is an input when p1.k == 0. */
+typedef struct vbm3d_clip {
+    float* noisy;
+    float* basic;
+    float* denoised;
+    const float* fflow; /* or NULL */
+    const float* bflow; /* or NULL */
+    unsigned W, H, F, C;
+    float sigma;
+    vbm3d_params p1, p2;
+    int color_space;
+} vbm3d_clip;
+
+/* Launch shape of the batched form.  Any choice gives bit-identical
+ * results; only the speed differs. */
+typedef struct vbm3d_batch_config {
+    unsigned chunk_frames;  /* reference frames per chunk; 0 = as many as the memory budget allows */
+    int waves_per_group;    /* waves per workgroup of the matching and filtering launches, 1..4; 0 = most that fit */
+    int grab;               /* reference patches a wave takes from the work queue at a time, >= 1 */
+    int groups_per_cu;      /* persistent workgroups per compute unit, >= 1 */
+    int use_graph;          /* capture the whole batch in a HIP graph and replay it on later identical calls */
+} vbm3d_batch_config;
+
+/* Denoises `nclips` clips of any sizes and parameters together: per step,
+ * per chunk of frames and per kernel class, one persistent launch whose
+ * workgroups take reference patches (or output tiles) of all clips from a
+ * global work queue.  With use_graph the whole sequence is captured once
+ * in a HIP graph and replayed on later calls with the same clips,
+ * buffers and configuration (not when a clip has flows).  cfg NULL means
+ * the configuration vbm3d_autotune_batch last chose for this context, or
+ * the defaults.  Each clip's output is bit-identical to vbm3d_run_device
+ * on that clip.  Enqueued after the work on `stream` (NULL = the
+ * context's stream); synchronised before return. */
+int vbm3d_run_batch(vbm3d_ctx* ctx, const vbm3d_clip* clips, int nclips,
+                    int ref_grid_nb, const vbm3d_batch_config* cfg,
+                    void* stream);
+
+/* Times candidate configurations of the batched form on these clips
+ * (coordinate search over waves_per_group, grab and groups_per_cu) and
+ * returns the fastest in *best; the context keeps it for later
+ * vbm3d_run_batch calls with cfg NULL.  The clips' buffers are restored
+ * after each trial, so inputs are left as they were. */
+int vbm3d_autotune_batch(vbm3d_ctx* ctx, const vbm3d_clip* clips,
+                         int nclips, int ref_grid_nb,
+                         vbm3d_batch_config* best, void* stream);
+
 /* Reference-patch grid of one frame in processing order (crop, row, col);
  * ind_initialize + subDivideTight (src/Utilities/Utilities.cpp:400-415,
  * src/Utilities/LibVideoT.hpp:903-961).  Returns the count, or -1. */

@@ -4,10 +4,11 @@ The denoising path of tehret/vbm3d (run_vbm3d, src/vbm3d.cpp:87-376) on
 hand-written gfx950 HIP kernels behind a C ABI (include/vbm3d_hip.h,
 libvbm3d_hip.so).  The Python API mirrors the reference's interface.
 """
-from .api import (BIOR, DCT, HAAR, HADAMARD, NONE, Context, Parameters, VBM3DError, block_match,
-                  denoise, grid, initialize_parameters_1, initialize_parameters_2, lib, run_device,
-                  run_vbm3d, step, step2_adjoint, sure)
+from .api import (BIOR, DCT, HAAR, HADAMARD, NONE, BatchConfig, BatchItem, Context, Parameters, VBM3DError,
+                  autotune_batch, block_match, denoise, grid, initialize_parameters_1, initialize_parameters_2,
+                  lib, run_batch, run_device, run_vbm3d, step, step2_adjoint, sure)
 
-__all__ = ["BIOR", "DCT", "HAAR", "HADAMARD", "NONE", "Context", "Parameters", "VBM3DError",
+__all__ = ["BIOR", "DCT", "HAAR", "HADAMARD", "NONE", "BatchConfig", "BatchItem", "Context", "Parameters",
+           "VBM3DError", "autotune_batch", "run_batch",
            "block_match", "denoise", "grid", "initialize_parameters_1", "initialize_parameters_2",
            "lib", "run_device", "run_vbm3d", "step", "step2_adjoint", "sure"]

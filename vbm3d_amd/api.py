@@ -47,6 +47,23 @@ class Parameters(C.Structure):
         return o
 
 
+class Clip(C.Structure):
+    """POD mirror of vbm3d_clip: one clip of a batch (device pointers)."""
+    _fields_ = [("noisy", C.c_void_p), ("basic", C.c_void_p), ("denoised", C.c_void_p),
+                ("fflow", C.c_void_p), ("bflow", C.c_void_p),
+                ("W", C.c_uint), ("H", C.c_uint), ("F", C.c_uint), ("C", C.c_uint),
+                ("sigma", C.c_float), ("p1", Parameters), ("p2", Parameters), ("color_space", C.c_int)]
+
+
+class BatchConfig(C.Structure):
+    """POD mirror of vbm3d_batch_config."""
+    _fields_ = [("chunk_frames", C.c_uint), ("waves_per_group", C.c_int), ("grab", C.c_int),
+                ("groups_per_cu", C.c_int), ("use_graph", C.c_int)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 _lib = None
 _lock = threading.Lock()
 
@@ -81,6 +98,10 @@ def lib():
             L.vbm3d_step2_adjoint.argtypes = [C.c_void_p, C.c_float, C.c_void_p, C.c_void_p, C.c_void_p,
                                               C.c_void_p, C.c_uint, C.c_uint, C.c_uint, C.c_uint, pp, C.c_int,
                                               C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+            L.vbm3d_run_batch.argtypes = [C.c_void_p, C.POINTER(Clip), C.c_int, C.c_int,
+                                          C.POINTER(BatchConfig), C.c_void_p]
+            L.vbm3d_autotune_batch.argtypes = [C.c_void_p, C.POINTER(Clip), C.c_int, C.c_int,
+                                               C.POINTER(BatchConfig), C.c_void_p]
             L.vbm3d_block_match.argtypes = [C.c_void_p, fp, C.c_uint, C.c_uint, C.c_uint, C.c_uint, fp, fp, pp,
                                             C.c_long, up, up, fp, up, up]
             L.vbm3d_step.argtypes = [C.c_void_p, C.c_int, C.c_float, fp, fp, fp, fp, C.c_uint, C.c_uint,
@@ -294,6 +315,68 @@ def sure(sigma_noise, noisy_t, prms_1, prms_2, sigma=None, basic_t=None, probe_s
     s2 = float(sigma_noise) ** 2
     res = float(((final.double() - y.double()) ** 2).sum()) / n
     return res - s2 + 2.0 * s2 * div / n, div, final
+
+
+class BatchItem:
+    """One clip of a batch: torch CUDA tensors [F][C][H][W] (float32,
+    contiguous) and its parameters.  `basic` and `denoised` are allocated
+    when not given; `noisy` is mutated like vid_noisy of run_vbm3d."""
+
+    def __init__(self, sigma, noisy, prms_1=None, prms_2=None, color_space=True, basic=None,
+                 denoised=None, fflow=None, bflow=None):
+        import torch
+        self.sigma = float(sigma)
+        self.noisy = noisy
+        self.basic = basic if basic is not None else torch.empty_like(noisy)
+        self.denoised = denoised if denoised is not None else torch.empty_like(noisy)
+        self.fflow, self.bflow = fflow, bflow
+        self.p1 = prms_1 if prms_1 is not None else initialize_parameters_1(sigma)
+        self.p2 = prms_2 if prms_2 is not None else initialize_parameters_2(sigma)
+        self.color_space = bool(color_space)
+
+    def as_clip(self) -> Clip:
+        for t in (self.noisy, self.basic, self.denoised):
+            if not t.is_contiguous() or t.shape != self.noisy.shape:
+                raise ValueError("batch buffers must be contiguous and of one shape")
+        F, Cc, H, W = self.noisy.shape
+        return Clip(_ptr(self.noisy).value, _ptr(self.basic).value, _ptr(self.denoised).value,
+                    None if self.fflow is None else self.fflow.data_ptr(),
+                    None if self.bflow is None else self.bflow.data_ptr(),
+                    W, H, F, Cc, self.sigma, self.p1, self.p2, int(self.color_space))
+
+
+def _clips(items):
+    arr = (Clip * len(items))()
+    for i, it in enumerate(items):
+        arr[i] = it.as_clip()
+    return arr
+
+
+def run_batch(items, ref_grid_nb=1, config: BatchConfig | dict | None = None, stream=None,
+              ctx: Context | None = None):
+    """Denoise a list of BatchItem together (vbm3d_run_batch): persistent
+    work-queue launches over all clips, captured in a HIP graph and
+    replayed on later identical calls.  config None uses the context's
+    autotuned configuration (or the defaults)."""
+    ctx = ctx or default_context()
+    cfg = None
+    if config is not None:
+        cfg = config if isinstance(config, BatchConfig) else BatchConfig(**config)
+    sp = None if stream is None else C.c_void_p(int(stream.cuda_stream))
+    arr = _clips(items)
+    _check(lib().vbm3d_run_batch(ctx.h, arr, len(items), int(ref_grid_nb),
+                                 None if cfg is None else C.byref(cfg), sp))
+
+
+def autotune_batch(items, ref_grid_nb=1, stream=None, ctx: Context | None = None) -> dict:
+    """Time launch shapes of the batched form on these clips; the context
+    keeps the fastest for run_batch(config=None).  Inputs are left as
+    they were."""
+    ctx = ctx or default_context()
+    best = BatchConfig()
+    sp = None if stream is None else C.c_void_p(int(stream.cuda_stream))
+    _check(lib().vbm3d_autotune_batch(ctx.h, _clips(items), len(items), int(ref_grid_nb), C.byref(best), sp))
+    return best.as_dict()
 
 
 def block_match(vid, prm, ref_idx, fflow=None, bflow=None, ctx=None):

@@ -251,13 +251,21 @@ struct vbm3d_ctx {
     hipStream_t stream = nullptr;
     DevBuf noisy, basic, den, ff, bf, num_raw, den_raw;  // host-buffer entry points
     DevBuf idx, nsx, nsim, dist, est, wgt, tabs, refxy, meta, gpos, rlist, scal, stats, counts;
-    size_t est_budget = (size_t)32 << 30;
+    size_t est_budget = (size_t)64 << 30;
     // kernel profiling (vbm3d_profile / vbm3d_profile_read)
     bool prof = false;    // bracket launches with events
     bool count = false;   // also accumulate work counters (atomics; not for timed runs)
     std::vector<ProfEvent> events;
     double contributions[3] = {0, 0, 0};  // output samples, per step
     double pix_per_cand[3] = {0, 0, 0};   // k*k*C of each step's matching
+    // batched form: rings shared by the clips of a batch, the plan (job
+    // arrays and queue prefixes), queue counters, and the captured graph
+    DevBuf b_idx, b_nsx, b_est, b_wgt, b_refxy, b_meta, b_gpos, b_tabs, b_plan, b_cnt, b_save;
+    hipGraph_t b_graph = nullptr;
+    hipGraphExec_t b_exec = nullptr;
+    std::string b_key;
+    bool b_tuned = false;       // b_tune holds vbm3d_autotune_batch's choice
+    vbm3d_batch_config b_tune{};
 };
 
 namespace {
@@ -488,6 +496,429 @@ hipError_t upload(DevBuf& b, const float* h, size_t n, hipStream_t s) {
 
 }  // namespace
 
+// ---------------------------------------------------------------------
+// Batched form.  The clips of a batch share the rings; per step, per chunk
+// of frames and per kernel class (patch size, channels) there is one
+// persistent launch whose workgroups take items -- reference patches for
+// matching and filtering, output tiles for aggregation -- from a global
+// work queue until it is empty.  Items of one launch are ordered by clip,
+// so each clip's per-patch results and each output sample are computed
+// exactly as in the per-clip path.
+// ---------------------------------------------------------------------
+namespace {
+
+struct BatchOp {
+    int kind;  // 0 match, 1 filter, 2 aggregate, 3 colour, 4 copy
+    int K = 0, C = 0, step = 0;
+    size_t jobs = 0, prefix = 0;  // byte offsets in the plan buffer
+    int njobs = 0, counter = 0, words = 0;
+    long total = 0;
+    float* v = nullptr;
+    const float* src = nullptr;
+    unsigned W = 0, H = 0, F = 0;
+    int fwd = 0;
+    size_t bytes = 0;
+};
+
+struct BatchPlan {
+    std::vector<char> blob;  // job arrays and queue prefixes, uploaded to b_plan
+    std::vector<BatchOp> ops;
+    int counters = 0;
+    size_t save_bytes = 0;
+};
+
+size_t blob_put(std::vector<char>& blob, const void* p, size_t n) {
+    size_t off = (blob.size() + 255) & ~(size_t)255;
+    blob.resize(off + n);
+    std::memcpy(blob.data() + off, p, n);
+    return off;
+}
+
+// One clip in one step.
+struct Member {
+    int clip;
+    const vbm3d_params* p;
+    unsigned K, C, N, nrf, Fr, ring = 0, maxflow = 0;
+    size_t per_frame = 0;  // ring bytes per frame of reference patches
+    size_t refxy = 0, meta = 0, gpos = 0, tab = 0;
+    int ncrop = 0;
+    size_t slot_base = 0, idx_base = 0, est_base = 0, wgt_base = 0;
+    unsigned t_done = 0;
+};
+
+int n_cus(int device) {
+    int n = 0;
+    if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || n <= 0) n = 256;
+    return n;
+}
+
+// Builds the whole batch as a list of launches.  Allocates the shared
+// rings (sized by the estimate budget) and uploads the grids and tables;
+// the job arrays go to plan.blob with device pointers already resolved
+// against b_plan-independent buffers.
+int plan_batch(vbm3d_ctx* ctx, const vbm3d_clip* clips, int n, int nb, const vbm3d_batch_config& cfg,
+               BatchPlan& plan, hipStream_t s) {
+    plan = BatchPlan();
+    // planning may move the shared buffers: a captured graph is stale from here on
+    HIP_TRY(hipStreamSynchronize(s));
+    if (ctx->b_exec) { (void)hipGraphExecDestroy(ctx->b_exec); ctx->b_exec = nullptr; }
+    if (ctx->b_graph) { (void)hipGraphDestroy(ctx->b_graph); ctx->b_graph = nullptr; }
+    ctx->b_key.clear();
+    std::vector<unsigned> maxflow(n, 0);
+    for (int i = 0; i < n; i++) {
+        const vbm3d_clip& c = clips[i];
+        if (!c.noisy || !c.basic || !c.denoised) return fail(VBM3D_ERR_ARG, "clip " + std::to_string(i) + ": null buffer");
+        if (c.C != 1 && c.C != 3) return fail(VBM3D_ERR_ARG, "clip " + std::to_string(i) + ": only 1 or 3 channels");
+        for (int st = 1; st <= 2; st++) {
+            const vbm3d_params& p = st == 1 ? c.p1 : c.p2;
+            if (p.k == 0) continue;
+            int rc = check(p, c.W, c.H, c.F, c.C, st);
+            if (rc) return fail(rc, "clip " + std::to_string(i) + ": " + g_err);
+        }
+        int rc = flow_absmax(ctx, c.fflow, c.bflow, c.W, c.H, c.F, s, &maxflow[i]);
+        if (rc) return rc;
+        if (c.C == 3 && c.color_space) plan.save_bytes += (size_t)c.W * c.H * c.F * 3 * 4 * (c.p1.k == 0 ? 2 : 1);
+    }
+    // host copies of grids and tables for both steps
+    std::vector<uint32_t> h_refxy;
+    std::vector<int> h_meta;
+    std::vector<uint16_t> h_gpos;
+    std::vector<Tabs> h_tabs;
+    std::vector<std::vector<Member>> steps(3);
+    for (int st = 1; st <= 2; st++)
+        for (int i = 0; i < n; i++) {
+            const vbm3d_clip& c = clips[i];
+            const vbm3d_params& p = st == 1 ? c.p1 : c.p2;
+            if (p.k == 0) continue;
+            GridDef g;
+            if (!build_grid(g, c.W, c.H, p.k, p.p, p.n, nb))
+                return fail(VBM3D_ERR_ARG, "clip " + std::to_string(i) + ": invalid reference grid");
+            Member m{};
+            m.clip = i;
+            m.p = &p;
+            m.K = p.k; m.C = c.C; m.N = p.N;
+            m.Fr = c.F - p.kt + 1;
+            m.maxflow = maxflow[i];
+            m.refxy = h_refxy.size();
+            m.meta = h_meta.size();
+            m.gpos = h_gpos.size();
+            m.ncrop = (int)g.rows.size();
+            for (size_t q = 0; q < g.rows.size(); q++) {
+                h_meta.push_back((int)(h_refxy.size() - m.refxy));
+                h_meta.push_back((int)g.rows[q].size());
+                h_meta.push_back((int)g.cols[q].size());
+                h_meta.push_back((int)(h_gpos.size() - m.gpos));
+                for (unsigned v : g.rows[q]) h_gpos.push_back((uint16_t)v);
+                h_meta.push_back((int)(h_gpos.size() - m.gpos));
+                for (unsigned v : g.cols[q]) h_gpos.push_back((uint16_t)v);
+                for (unsigned y : g.rows[q])
+                    for (unsigned x : g.cols[q]) h_refxy.push_back(x | (y << 16));
+            }
+            m.nrf = (unsigned)(h_refxy.size() - m.refxy);
+            m.per_frame = (size_t)m.nrf * (m.C * m.N * m.K * m.K * 4 + m.N * 4 + 4 + m.C * 4);
+            Tabs t;
+            fill_tabs(t, m.K);
+            m.tab = h_tabs.size();
+            h_tabs.push_back(t);
+            steps[st].push_back(m);
+        }
+    // shared rings: sub-batches of members whose smallest rings fit the budget,
+    // then the largest chunk that fits
+    struct Sub { size_t b, e; unsigned S; };
+    std::vector<std::vector<Sub>> subs(3);
+    size_t need_slots = 0, need_idx = 0, need_est = 0, need_wgt = 0;
+    for (int st = 1; st <= 2; st++) {
+        std::vector<Member>& ms = steps[st];
+        size_t i = 0;
+        while (i < ms.size()) {
+            size_t j = i, used = 0;
+            while (j < ms.size()) {
+                const vbm3d_params& p = *ms[j].p;
+                const size_t r1 = std::min<size_t>(ms[j].Fr, 1 + 2 * p.Nf) * ms[j].per_frame;
+                if (j > i && used + r1 > ctx->est_budget) break;
+                used += r1;
+                j++;
+            }
+            unsigned maxFr = 0;
+            for (size_t q = i; q < j; q++) maxFr = std::max(maxFr, ms[q].Fr);
+            auto ring_of = [&](const Member& m, unsigned S) -> unsigned {
+                return S >= m.Fr ? m.Fr : std::min<unsigned>(m.Fr, S + 2 * m.p->Nf);
+            };
+            auto bytes_at = [&](unsigned S) {
+                size_t b = 0;
+                for (size_t q = i; q < j; q++) b += (size_t)ring_of(ms[q], S) * ms[q].per_frame;
+                return b;
+            };
+            unsigned S = cfg.chunk_frames;
+            if (S == 0) {
+                S = maxFr;
+                while (S > 1 && bytes_at(S) > ctx->est_budget) S--;
+            }
+            S = std::max(1u, std::min(S, maxFr));
+            size_t sb = 0, ib = 0, eb = 0, wb = 0;
+            for (size_t q = i; q < j; q++) {
+                Member& m = ms[q];
+                m.ring = ring_of(m, S);
+                m.slot_base = sb;
+                m.idx_base = ib;
+                m.est_base = eb;
+                m.wgt_base = wb;
+                const size_t slots = (size_t)m.ring * m.nrf;
+                sb += slots;
+                ib += slots * m.N;
+                eb += slots * m.C * m.N * m.K * m.K;
+                wb += slots * m.C;
+            }
+            if (((size_t)sb * kMaxN) >> 32) return fail(VBM3D_ERR_UNSUPPORTED, "too many reference patches in flight");
+            need_slots = std::max(need_slots, sb);
+            need_idx = std::max(need_idx, ib);
+            need_est = std::max(need_est, eb);
+            need_wgt = std::max(need_wgt, wb);
+            subs[st].push_back(Sub{i, j, S});
+            i = j;
+        }
+    }
+    HIP_TRY(ensure(ctx->b_idx, std::max<size_t>(need_idx, 1) * 4));
+    HIP_TRY(ensure(ctx->b_nsx, std::max<size_t>(need_slots, 1) * 4));
+    HIP_TRY(ensure(ctx->b_est, std::max<size_t>(need_est, 1) * 4));
+    HIP_TRY(ensure(ctx->b_wgt, std::max<size_t>(need_wgt, 1) * 4));
+    HIP_TRY(ensure(ctx->b_refxy, std::max<size_t>(h_refxy.size(), 1) * 4));
+    HIP_TRY(ensure(ctx->b_meta, std::max<size_t>(h_meta.size(), 1) * 4));
+    HIP_TRY(ensure(ctx->b_gpos, std::max<size_t>(h_gpos.size(), 1) * 2));
+    HIP_TRY(ensure(ctx->b_tabs, std::max<size_t>(h_tabs.size(), 1) * sizeof(Tabs)));
+    if (!h_refxy.empty()) HIP_TRY(hipMemcpyAsync(ctx->b_refxy.p, h_refxy.data(), h_refxy.size() * 4, hipMemcpyHostToDevice, s));
+    if (!h_meta.empty()) HIP_TRY(hipMemcpyAsync(ctx->b_meta.p, h_meta.data(), h_meta.size() * 4, hipMemcpyHostToDevice, s));
+    if (!h_gpos.empty()) HIP_TRY(hipMemcpyAsync(ctx->b_gpos.p, h_gpos.data(), h_gpos.size() * 2, hipMemcpyHostToDevice, s));
+    if (!h_tabs.empty()) HIP_TRY(hipMemcpyAsync(ctx->b_tabs.p, h_tabs.data(), h_tabs.size() * sizeof(Tabs), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipStreamSynchronize(s));  // the host vectors go out of scope
+
+    uint32_t* d_idx = (uint32_t*)ctx->b_idx.p;
+    uint32_t* d_nsx = (uint32_t*)ctx->b_nsx.p;
+    float* d_est = (float*)ctx->b_est.p;
+    float* d_wgt = (float*)ctx->b_wgt.p;
+    const uint32_t* d_refxy = (const uint32_t*)ctx->b_refxy.p;
+    const int* d_meta = (const int*)ctx->b_meta.p;
+    const uint16_t* d_gpos = (const uint16_t*)ctx->b_gpos.p;
+    const Tabs* d_tabs = (const Tabs*)ctx->b_tabs.p;
+
+    auto colour = [&](float* v, const vbm3d_clip& c, bool fwd) {
+        BatchOp op;
+        op.kind = 3; op.v = v; op.W = c.W; op.H = c.H; op.F = c.F; op.fwd = fwd ? 1 : 0;
+        plan.ops.push_back(op);
+    };
+    for (int i = 0; i < n; i++)
+        if (clips[i].C == 3 && clips[i].color_space) colour(clips[i].p1.k > 0 ? clips[i].noisy : clips[i].basic, clips[i], true);
+
+    for (int st = 1; st <= 2; st++) {
+        std::vector<Member>& ms = steps[st];
+        for (const Sub& sub : subs[st]) {
+            unsigned maxFr = 0;
+            for (size_t q = sub.b; q < sub.e; q++) maxFr = std::max(maxFr, ms[q].Fr);
+            for (unsigned a = 0; a < maxFr; a += sub.S) {
+                // per class (K, C): matching and filtering jobs of this chunk
+                struct Cls { int K, C; std::vector<BMArgs> bm; std::vector<FArgs> fa; std::vector<long> pre; int wb = 0, wf = 0; };
+                std::vector<Cls> cls;
+                std::vector<AArgs> ag1, ag3;
+                std::vector<long> pre1{0}, pre3{0};
+                for (size_t q = sub.b; q < sub.e; q++) {
+                    Member& m = ms[q];
+                    const vbm3d_clip& c = clips[m.clip];
+                    const vbm3d_params& p = *m.p;
+                    float sig[3] = {0, 0, 0};
+                    sigma_rows(c.sigma, c.C, c.color_space != 0, sig);
+                    const unsigned wh = c.W * c.H, whc = wh * c.C;
+                    if (a < m.Fr) {
+                        const unsigned e = std::min(m.Fr, a + sub.S);
+                        BMArgs bm{};
+                        bm.vid = st == 1 ? c.noisy : c.basic;
+                        bm.ff = c.fflow; bm.bf = c.bflow;
+                        bm.W = c.W; bm.H = c.H; bm.F = c.F; bm.C = c.C; bm.wh = wh; bm.whc = whc;
+                        bm.nrf = m.nrf;
+                        bm.ref_xy = d_refxy + m.refxy;
+                        bm.ref_list = nullptr;
+                        bm.r_begin = (long)a * m.nrf;
+                        bm.r_end = (long)e * m.nrf;
+                        bm.ring = m.ring;
+                        bm.Ns = p.Ns; bm.Npr = p.Npr; bm.Nb = p.Nb; bm.Nf = p.Nf; bm.N = p.N; bm.kt = p.kt;
+                        bm.d = p.d; bm.tau = p.tau;
+                        bm.idx_out = d_idx + m.idx_base;
+                        bm.dist_out = nullptr;
+                        bm.nsim_out = nullptr;
+                        bm.nsx_out = d_nsx + m.slot_base;
+                        bm.count_out = nullptr;
+                        FArgs fa{};
+                        fa.noisy = c.noisy;
+                        fa.basic = c.basic;
+                        fa.W = c.W; fa.H = c.H; fa.F = c.F; fa.C = c.C; fa.wh = wh; fa.whc = whc;
+                        fa.nrf = m.nrf; fa.ring = m.ring;
+                        fa.r_begin = bm.r_begin; fa.r_end = bm.r_end;
+                        fa.idx = bm.idx_out; fa.nsx = bm.nsx_out;
+                        fa.N = p.N; fa.T2D = p.T_2D; fa.T3D = p.T_3D; fa.lambda = p.lambda3D;
+                        for (int k = 0; k < 3; k++) fa.sig[k] = sig[k];
+                        fa.tabs = d_tabs + m.tab;
+                        fa.est = d_est + m.est_base;
+                        fa.wgt = d_wgt + m.wgt_base;
+                        fa.gy = nullptr; fa.den = nullptr;
+                        Cls* cl = nullptr;
+                        for (Cls& x : cls)
+                            if (x.K == (int)m.K && x.C == (int)m.C) cl = &x;
+                        if (!cl) {
+                            cls.push_back(Cls{(int)m.K, (int)m.C, {}, {}, {0}});
+                            cl = &cls.back();
+                        }
+                        cl->bm.push_back(bm);
+                        cl->fa.push_back(fa);
+                        cl->pre.push_back(cl->pre.back() + (bm.r_end - bm.r_begin));
+                        cl->wb = std::max(cl->wb, bm_lds_words((int)m.K, (int)m.C, p.Ns, p.Npr, p.Nb));
+                        cl->wf = std::max(cl->wf, filter_words((int)m.K, (int)m.C, p.N, st));
+                    }
+                    const unsigned e = std::min(m.Fr, a + sub.S);
+                    const unsigned ready = e >= m.Fr ? c.F : (e > p.Nf ? e - p.Nf : 0);
+                    if (ready > m.t_done) {
+                        AArgs ag{};
+                        ag.W = c.W; ag.H = c.H; ag.F = c.F; ag.C = c.C; ag.wh = wh; ag.whc = whc;
+                        ag.K = m.K; ag.N = p.N; ag.Nf = p.Nf; ag.kt = p.kt;
+                        ag.nrf = m.nrf; ag.ring = m.ring; ag.Fr = m.Fr;
+                        ag.ncrop = m.ncrop;
+                        ag.crop_meta = d_meta + m.meta;
+                        ag.grid_pos = d_gpos + m.gpos;
+                        ag.idx = d_idx + m.idx_base;
+                        ag.nsx = d_nsx + m.slot_base;
+                        ag.est = d_est + m.est_base;
+                        ag.wgt = d_wgt + m.wgt_base;
+                        ag.tabs = d_tabs + m.tab;
+                        ag.R = (int)((p.Ns - 1) / 2 + p.Nf * ((p.Npr - 1) / 2 + m.maxflow));
+                        ag.divW = make_div32(c.W);
+                        ag.t_begin = m.t_done;
+                        ag.fallback = st == 1 ? c.noisy : (nb == 1 ? c.basic : c.noisy);
+                        ag.out = st == 1 ? c.basic : c.denoised;
+                        ag.num_out = nullptr;
+                        ag.den_out = nullptr;
+                        ag.raw = 0;
+                        const long tiles = (long)((c.W + kAggTile - 1) / kAggTile) * ((c.H + kAggTile - 1) / kAggTile) *
+                                           (ready - m.t_done);
+                        if (c.C == 1) { ag1.push_back(ag); pre1.push_back(pre1.back() + tiles); }
+                        else { ag3.push_back(ag); pre3.push_back(pre3.back() + tiles); }
+                        m.t_done = ready;
+                    }
+                }
+                for (Cls& cl : cls) {
+                    BatchOp op;
+                    op.kind = 0; op.K = cl.K; op.C = cl.C; op.step = st;
+                    op.jobs = blob_put(plan.blob, cl.bm.data(), cl.bm.size() * sizeof(BMArgs));
+                    op.prefix = blob_put(plan.blob, cl.pre.data(), cl.pre.size() * sizeof(long));
+                    op.njobs = (int)cl.bm.size(); op.total = cl.pre.back(); op.words = cl.wb;
+                    op.counter = plan.counters++;
+                    plan.ops.push_back(op);
+                }
+                for (Cls& cl : cls) {
+                    BatchOp op;
+                    op.kind = 1; op.K = cl.K; op.C = cl.C; op.step = st;
+                    op.jobs = blob_put(plan.blob, cl.fa.data(), cl.fa.size() * sizeof(FArgs));
+                    op.prefix = blob_put(plan.blob, cl.pre.data(), cl.pre.size() * sizeof(long));
+                    op.njobs = (int)cl.fa.size(); op.total = cl.pre.back(); op.words = cl.wf;
+                    op.counter = plan.counters++;
+                    plan.ops.push_back(op);
+                }
+                for (int cc : {1, 3}) {
+                    std::vector<AArgs>& v = cc == 1 ? ag1 : ag3;
+                    std::vector<long>& pre = cc == 1 ? pre1 : pre3;
+                    if (v.empty()) continue;
+                    BatchOp op;
+                    op.kind = 2; op.C = cc; op.step = st;
+                    op.jobs = blob_put(plan.blob, v.data(), v.size() * sizeof(AArgs));
+                    op.prefix = blob_put(plan.blob, pre.data(), pre.size() * sizeof(long));
+                    op.njobs = (int)v.size(); op.total = pre.back();
+                    op.counter = plan.counters++;
+                    plan.ops.push_back(op);
+                }
+            }
+        }
+        if (st == 2)
+            for (int i = 0; i < n; i++)
+                if (clips[i].p2.k == 0) {
+                    BatchOp op;
+                    op.kind = 4; op.v = clips[i].denoised; op.src = clips[i].basic;
+                    op.bytes = (size_t)clips[i].W * clips[i].H * clips[i].F * clips[i].C * 4;
+                    plan.ops.push_back(op);
+                }
+    }
+    for (int i = 0; i < n; i++)
+        if (clips[i].C == 3 && clips[i].color_space) {
+            colour(clips[i].denoised, clips[i], false);
+            colour(clips[i].noisy, clips[i], false);
+            colour(clips[i].basic, clips[i], false);
+        }
+    HIP_TRY(ensure(ctx->b_plan, std::max<size_t>(plan.blob.size(), 1)));
+    if (!plan.blob.empty()) HIP_TRY(hipMemcpy(ctx->b_plan.p, plan.blob.data(), plan.blob.size(), hipMemcpyHostToDevice));
+    HIP_TRY(ensure(ctx->b_cnt, std::max<size_t>(plan.counters, 1) * sizeof(unsigned long long)));
+    return VBM3D_OK;
+}
+
+// Enqueues the plan on s (also what a graph capture records).
+int exec_batch(vbm3d_ctx* ctx, const BatchPlan& plan, const vbm3d_batch_config& cfg, hipStream_t s) {
+    char* blob = (char*)ctx->b_plan.p;
+    unsigned long long* cnt = (unsigned long long*)ctx->b_cnt.p;
+    const int cus = n_cus(ctx->device);
+    HIP_TRY(launch_zero_u64(cnt, plan.counters, s));
+    for (const BatchOp& op : plan.ops) {
+        if (op.kind <= 2) {
+            WorkQueue q;
+            q.counter = cnt + op.counter;
+            q.item_begin = (const long*)(blob + op.prefix);
+            q.njobs = op.njobs;
+            q.total = op.total;
+            q.grab = op.kind == 2 ? 1 : std::max(1, cfg.grab);
+            if (op.kind == 2) {
+                const long blocks = std::min<long>(op.total, (long)cfg.groups_per_cu * cus);
+                HIP_TRY(launch_aggregate_queue(op.C, (const AArgs*)(blob + op.jobs), q, (int)blocks, s));
+                continue;
+            }
+            int waves = cfg.waves_per_group > 0 ? std::min(cfg.waves_per_group, 4) : 4;
+            while (waves > 1 && (size_t)op.words * 4 * waves > 64 * 1024) waves--;
+            if ((size_t)op.words * 4 > 64 * 1024) return fail(VBM3D_ERR_UNSUPPORTED, "patch working set above 64 KB of LDS");
+            const long need = (op.total + (long)q.grab * waves - 1) / ((long)q.grab * waves);
+            const long blocks = std::min<long>(need, (long)cfg.groups_per_cu * cus);
+            if (op.kind == 0)
+                HIP_TRY(launch_bm_queue(op.K, op.C, (const BMArgs*)(blob + op.jobs), q, op.words, waves, (int)blocks, s));
+            else
+                HIP_TRY(launch_filter_queue(op.K, op.C, op.step, (const FArgs*)(blob + op.jobs), q, op.words, waves,
+                                            (int)blocks, s));
+        } else if (op.kind == 3) {
+            HIP_TRY(launch_color(op.v, op.W, op.H, op.F, op.fwd != 0, s));
+        } else {
+            HIP_TRY(hipMemcpyAsync(op.v, op.src, op.bytes, hipMemcpyDeviceToDevice, s));
+        }
+    }
+    return VBM3D_OK;
+}
+
+vbm3d_batch_config default_batch_config() {
+    vbm3d_batch_config c;
+    c.chunk_frames = 0;
+    c.waves_per_group = 0;
+    c.grab = 1;
+    c.groups_per_cu = 8;
+    c.use_graph = 1;
+    return c;
+}
+
+int check_batch_config(const vbm3d_batch_config& c) {
+    if (c.waves_per_group < 0 || c.waves_per_group > 4) return fail(VBM3D_ERR_ARG, "waves_per_group must be in [0, 4]");
+    if (c.grab < 1 || c.grab > 64) return fail(VBM3D_ERR_ARG, "grab must be in [1, 64]");
+    if (c.groups_per_cu < 1 || c.groups_per_cu > 64) return fail(VBM3D_ERR_ARG, "groups_per_cu must be in [1, 64]");
+    return VBM3D_OK;
+}
+
+std::string batch_key(const vbm3d_clip* clips, int n, int nb, const vbm3d_batch_config& cfg) {
+    std::string k((const char*)clips, (size_t)n * sizeof(vbm3d_clip));
+    k.append((const char*)&nb, sizeof nb);
+    k.append((const char*)&cfg, sizeof cfg);
+    return k;
+}
+
+}  // namespace
+
 extern "C" {
 
 const char* vbm3d_last_error(void) { return g_err.c_str(); }
@@ -514,8 +945,12 @@ void vbm3d_destroy(vbm3d_ctx* c) {
     (void)hipStreamSynchronize(c->stream);
     for (DevBuf* b : {&c->noisy, &c->basic, &c->den, &c->ff, &c->bf, &c->num_raw, &c->den_raw,
                       &c->idx, &c->nsx, &c->nsim, &c->dist, &c->est, &c->wgt, &c->tabs,
-                      &c->refxy, &c->meta, &c->gpos, &c->rlist, &c->scal})
+                      &c->refxy, &c->meta, &c->gpos, &c->rlist, &c->scal, &c->stats, &c->counts,
+                      &c->b_idx, &c->b_nsx, &c->b_est, &c->b_wgt, &c->b_refxy, &c->b_meta, &c->b_gpos,
+                      &c->b_tabs, &c->b_plan, &c->b_cnt, &c->b_save})
         if (b->p) (void)hipFree(b->p);
+    if (c->b_exec) (void)hipGraphExecDestroy(c->b_exec);
+    if (c->b_graph) (void)hipGraphDestroy(c->b_graph);
     (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -805,6 +1240,140 @@ void vbm3d_selftest_partial_sort(float* dist, uint32_t* idx, unsigned len, unsig
         dist[i] = v[i].d;
         idx[i] = v[i].i;
     }
+}
+
+int vbm3d_run_batch(vbm3d_ctx* ctx, const vbm3d_clip* clips, int nclips, int ref_grid_nb,
+                    const vbm3d_batch_config* cfgp, void* stream) {
+    if (!ctx || (!clips && nclips > 0) || nclips < 0) return fail(VBM3D_ERR_ARG, "invalid argument");
+    if (nclips == 0) return VBM3D_OK;
+    HIP_TRY(hipSetDevice(ctx->device));
+    const vbm3d_batch_config cfg = cfgp ? *cfgp : (ctx->b_tuned ? ctx->b_tune : default_batch_config());
+    int rc = check_batch_config(cfg);
+    if (rc) return rc;
+    // the batch runs on the context's stream, after the caller's work
+    if (stream) HIP_TRY(hipStreamSynchronize((hipStream_t)stream));
+    hipStream_t s = ctx->stream;
+    bool flows = false;
+    for (int i = 0; i < nclips; i++) flows = flows || clips[i].fflow || clips[i].bflow;
+    const std::string key = batch_key(clips, nclips, ref_grid_nb, cfg);
+    if (cfg.use_graph && !flows && ctx->b_exec && key == ctx->b_key) {
+        HIP_TRY(hipGraphLaunch(ctx->b_exec, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        return VBM3D_OK;
+    }
+    BatchPlan plan;
+    rc = plan_batch(ctx, clips, nclips, ref_grid_nb, cfg, plan, s);
+    if (rc) return rc;
+    if (!cfg.use_graph || flows) {
+        rc = exec_batch(ctx, plan, cfg, s);
+        if (rc) return rc;
+        HIP_TRY(hipStreamSynchronize(s));
+        return VBM3D_OK;
+    }
+    HIP_TRY(hipStreamSynchronize(s));
+    HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    rc = exec_batch(ctx, plan, cfg, s);
+    hipGraph_t g = nullptr;
+    const hipError_t ce = hipStreamEndCapture(s, &g);
+    if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
+    if (ce != hipSuccess) return fail(VBM3D_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ce));
+    hipGraphExec_t ex = nullptr;
+    const hipError_t ie = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    if (ie != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        return fail(VBM3D_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ie));
+    }
+    ctx->b_graph = g;
+    ctx->b_exec = ex;
+    ctx->b_key = key;
+    HIP_TRY(hipGraphLaunch(ex, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return VBM3D_OK;
+}
+
+int vbm3d_autotune_batch(vbm3d_ctx* ctx, const vbm3d_clip* clips, int nclips, int ref_grid_nb,
+                         vbm3d_batch_config* best, void* stream) {
+    if (!ctx || !clips || nclips <= 0 || !best) return fail(VBM3D_ERR_ARG, "invalid argument");
+    HIP_TRY(hipSetDevice(ctx->device));
+    if (stream) HIP_TRY(hipStreamSynchronize((hipStream_t)stream));
+    hipStream_t s = ctx->stream;
+    vbm3d_batch_config cur = default_batch_config();
+    cur.use_graph = 0;
+    BatchPlan plan;
+    int rc = plan_batch(ctx, clips, nclips, ref_grid_nb, cur, plan, s);
+    if (rc) return rc;
+    // inputs the trials mutate (colour round trips) are saved and restored
+    HIP_TRY(ensure(ctx->b_save, std::max<size_t>(plan.save_bytes, 1)));
+    std::vector<std::pair<float*, size_t>> saved;
+    size_t off = 0;
+    for (int i = 0; i < nclips; i++) {
+        const vbm3d_clip& c = clips[i];
+        if (!(c.C == 3 && c.color_space)) continue;
+        const size_t bytes = (size_t)c.W * c.H * c.F * 3 * 4;
+        for (float* v : {c.noisy, c.basic}) {
+            if (v == c.basic && c.p1.k != 0) continue;
+            HIP_TRY(hipMemcpyAsync((char*)ctx->b_save.p + off, v, bytes, hipMemcpyDeviceToDevice, s));
+            saved.push_back({v, off});
+            off += bytes;
+        }
+    }
+    auto restore = [&]() -> int {
+        for (int i = 0, k = 0; i < nclips; i++) {
+            const vbm3d_clip& c = clips[i];
+            if (!(c.C == 3 && c.color_space)) continue;
+            const size_t bytes = (size_t)c.W * c.H * c.F * 3 * 4;
+            for (float* v : {c.noisy, c.basic}) {
+                if (v == c.basic && c.p1.k != 0) continue;
+                HIP_TRY(hipMemcpyAsync(v, (char*)ctx->b_save.p + saved[k].second, bytes, hipMemcpyDeviceToDevice, s));
+                k++;
+            }
+        }
+        return VBM3D_OK;
+    };
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    auto trial = [&](const vbm3d_batch_config& c, float* ms) -> int {
+        int r = restore();
+        if (r) return r;
+        HIP_TRY(hipEventRecord(e0, s));
+        r = exec_batch(ctx, plan, c, s);
+        if (r) return r;
+        HIP_TRY(hipEventRecord(e1, s));
+        HIP_TRY(hipEventSynchronize(e1));
+        HIP_TRY(hipEventElapsedTime(ms, e0, e1));
+        return VBM3D_OK;
+    };
+    float best_ms = 0;
+    rc = trial(cur, &best_ms);  // also warms up
+    if (!rc) rc = trial(cur, &best_ms);
+    const int waves_c[] = {1, 2, 4};
+    const int grab_c[] = {1, 2, 4, 8};
+    const int gpc_c[] = {2, 4, 8, 16};
+    for (int dim = 0; dim < 3 && !rc; dim++) {
+        const int* vals = dim == 0 ? waves_c : dim == 1 ? grab_c : gpc_c;
+        const int nv = dim == 0 ? 3 : 4;
+        vbm3d_batch_config keep = cur;
+        for (int v = 0; v < nv && !rc; v++) {
+            vbm3d_batch_config c = keep;
+            if (dim == 0) c.waves_per_group = vals[v];
+            else if (dim == 1) c.grab = vals[v];
+            else c.groups_per_cu = vals[v];
+            float ms = 0;
+            rc = trial(c, &ms);
+            if (!rc && ms < best_ms) { best_ms = ms; cur = c; }
+        }
+    }
+    if (!rc) rc = restore();
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(s));
+    cur.use_graph = 1;
+    *best = cur;
+    ctx->b_tuned = true;
+    ctx->b_tune = cur;
+    return VBM3D_OK;
 }
 
 }  // extern "C"

@@ -143,6 +143,15 @@ struct AArgs {
     int raw;                    // 1: plain sums of est (no Kaiser window, no weight)
 };
 
+// Global work queue of a batched (persistent) launch.
+struct WorkQueue {
+    unsigned long long* counter;  // zeroed before the launch
+    const long* item_begin;       // [njobs + 1] first item of each job
+    int njobs;
+    long total;                   // item_begin[njobs]
+    int grab;                     // items taken per atomic
+};
+
 // Launchers (kernels.hip).  Return hipSuccess or the launch error.
 hipError_t launch_bm(int K, const BMArgs& a, hipStream_t s);
 hipError_t launch_filter(int K, int step, const FArgs& a, hipStream_t s);
@@ -150,6 +159,14 @@ hipError_t launch_aggregate(int C, const AArgs& a, unsigned frames, hipStream_t 
 hipError_t launch_color(float* v, unsigned W, unsigned H, unsigned F, bool forward,
                         hipStream_t s);
 hipError_t launch_filter_adjoint(int K, const FArgs& a, hipStream_t s);
+hipError_t launch_bm_queue(int K, int C, const BMArgs* jobs, const WorkQueue& q, int words, int waves,
+                           int blocks, hipStream_t s);
+hipError_t launch_filter_queue(int K, int C, int step, const FArgs* jobs, const WorkQueue& q, int words,
+                               int waves, int blocks, hipStream_t s);
+hipError_t launch_aggregate_queue(int C, const AArgs* jobs, const WorkQueue& q, int blocks, hipStream_t s);
+hipError_t launch_zero_u64(unsigned long long* p, int n, hipStream_t s);
+int bm_lds_words(int K, int C, int Ns, int Npr, int Nb);  // per wave, 4-byte words
+int filter_words(int K, int C, int N, int step);
 hipError_t launch_adjoint_fallback(float* gz, const float* gy, const float* den, size_t n, hipStream_t s);
 hipError_t launch_sum_counts(const uint32_t* counts, long n, unsigned long long* stats, hipStream_t s);
 hipError_t launch_flow_absmax(const float* fl, size_t n, unsigned* out, hipStream_t s);

@@ -32,6 +32,27 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// A global work queue: items [0, total) of a batched launch, taken `grab`
+// at a time through an atomic counter (zeroed before the launch); job j
+// owns items [item_begin[j], item_begin[j+1]).
+__device__ __forceinline__ long queue_take(const WorkQueue& q, int lane) {
+    unsigned long long v = 0;
+    if (lane == 0) v = atomicAdd(q.counter, (unsigned long long)q.grab);
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
+    return (long)(((unsigned long long)hi << 32) | lo);
+}
+
+__device__ __forceinline__ int queue_job(const WorkQueue& q, long it) {
+    int lo = 0, hi = q.njobs - 1;  // largest j with item_begin[j] <= it
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (q.item_begin[mid] <= it) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
 // Monotone map float -> uint32 (for finite inputs); -0 and +0 map alike so
 // that equal floats give equal keys.
 __device__ __forceinline__ unsigned fkey(float f) {
@@ -109,15 +130,8 @@ __device__ void wave_select(const float* ld, const uint32_t* li, int m, int mid,
 // reference patch (also in LDS), in the reference's (c, y, x) order.
 // ---------------------------------------------------------------------
 template <int K, int C>
-__global__ __launch_bounds__(256) void k_bm(BMArgs a) {
-    extern __shared__ float smem[];
+__device__ __forceinline__ void bm_one(const BMArgs& a, const long r, float* base, const int lane) {
     const BMLay L = bm_layout(K, C, a.Ns, a.Npr, a.Nb);
-    const int lane = threadIdx.x & 63;
-    const int w = threadIdx.x >> 6;
-    const int nw = blockDim.x >> 6;
-    const long r = a.r_begin + (long)blockIdx.x * nw + w;
-    if (r >= a.r_end) return;  // wave-uniform
-    float* base = smem + (size_t)w * L.total;
     float* tile = base + L.tile;
     float* s_d = base + L.sd;
     uint32_t* s_i = (uint32_t*)(base + L.si);
@@ -400,6 +414,37 @@ __global__ __launch_bounds__(256) void k_bm(BMArgs a) {
     }
 }
 
+template <int K, int C>
+__global__ __launch_bounds__(256) void k_bm(BMArgs a) {
+    extern __shared__ float smem[];
+    const BMLay L = bm_layout(K, C, a.Ns, a.Npr, a.Nb);
+    const int lane = threadIdx.x & 63;
+    const int w = threadIdx.x >> 6;
+    const int nw = blockDim.x >> 6;
+    const long r = a.r_begin + (long)blockIdx.x * nw + w;
+    if (r >= a.r_end) return;  // wave-uniform
+    bm_one<K, C>(a, r, smem + (size_t)w * L.total, lane);
+}
+
+// Persistent form over a batch: every wave takes `grab` items at a time
+// from the queue counter until the queue is empty; item -> (job, patch).
+template <int K, int C>
+__global__ __launch_bounds__(256) void k_bm_queue(const BMArgs* __restrict__ jobs, WorkQueue q, int words) {
+    extern __shared__ float smem[];
+    const int lane = threadIdx.x & 63;
+    float* base = smem + (size_t)(threadIdx.x >> 6) * words;
+    for (;;) {
+        const long it0 = queue_take(q, lane);
+        if (it0 >= q.total) break;
+        const long it1 = min(q.total, it0 + (long)q.grab);
+        for (long it = it0; it < it1; it++) {
+            const int j = queue_job(q, it);
+            const BMArgs& a = jobs[j];
+            bm_one<K, C>(a, a.r_begin + (it - q.item_begin[j]), base, lane);
+        }
+    }
+}
+
 // ---------------------------------------------------------------------
 // Group filtering.
 // ---------------------------------------------------------------------
@@ -548,18 +593,11 @@ __host__ __device__ inline int filter_lds_words(int K, int C, int N, int step) {
 }
 
 template <int K, int C, int STEP>
-__global__ __launch_bounds__(256) void k_filter(FArgs a) {
+__device__ __forceinline__ void filter_one(const FArgs& a, const long r, float* X, const int lane) {
     constexpr int K2 = K * K;
     constexpr int PB = 64 / K;           // patches per lane batch
     constexpr int XS = xstride(K), XP = K + 1;
-    extern __shared__ float smem[];
-    const int lane = threadIdx.x & 63;
-    const int w = threadIdx.x >> 6;
-    const int nw = blockDim.x >> 6;
-    const long r = a.r_begin + (long)blockIdx.x * nw + w;
-    if (r >= a.r_end) return;
-    const int words = filter_lds_words(K, C, a.N, STEP);
-    float* X = smem + (size_t)w * words;          // [PB][XS] transposes
+    // X: [PB][XS] transposes, then the group buffers
     float* zg = X + PB * XS;                       // [N][K2]
     float* eg = zg + a.N * K2;                     // [N][K2] (step 2)
     float* v2 = eg + a.N * K2;                     // [C][K2*N] (step 2)
@@ -695,6 +733,34 @@ __global__ __launch_bounds__(256) void k_filter(FArgs a) {
         for (int c = 1; c < C; c++)
             if (lane == c) wsel = weight[c];
         a.wgt[(size_t)slot * C + lane] = wsel;
+    }
+}
+
+template <int K, int C, int STEP>
+__global__ __launch_bounds__(256) void k_filter(FArgs a) {
+    extern __shared__ float smem[];
+    const int lane = threadIdx.x & 63;
+    const int w = threadIdx.x >> 6;
+    const int nw = blockDim.x >> 6;
+    const long r = a.r_begin + (long)blockIdx.x * nw + w;
+    if (r >= a.r_end) return;
+    filter_one<K, C, STEP>(a, r, smem + (size_t)w * filter_lds_words(K, C, a.N, STEP), lane);
+}
+
+template <int K, int C, int STEP>
+__global__ __launch_bounds__(256) void k_filter_queue(const FArgs* __restrict__ jobs, WorkQueue q, int words) {
+    extern __shared__ float smem[];
+    const int lane = threadIdx.x & 63;
+    float* X = smem + (size_t)(threadIdx.x >> 6) * words;
+    for (;;) {
+        const long it0 = queue_take(q, lane);
+        if (it0 >= q.total) break;
+        const long it1 = min(q.total, it0 + (long)q.grab);
+        for (long it = it0; it < it1; it++) {
+            const int j = queue_job(q, it);
+            const FArgs& a = jobs[j];
+            filter_one<K, C, STEP>(a, a.r_begin + (it - q.item_begin[j]), X, lane);
+        }
     }
 }
 
@@ -941,7 +1007,7 @@ __device__ __forceinline__ int lower_bound_u16(const uint16_t* a, int n, int v) 
 // in list order: every output sample sees its contributions in exactly the
 // reference's order.
 template <int C>
-__global__ __launch_bounds__(256) void k_aggregate(AArgs a) {
+__device__ __forceinline__ void agg_tile(const AArgs& a, const int bx, const int by, const unsigned bz) {
     constexpr int TS = kAggTile;
     constexpr int QS = TS / 2;
     __shared__ uint32_t L_sn[256 * kMaxN];
@@ -950,8 +1016,8 @@ __global__ __launch_bounds__(256) void k_aggregate(AArgs a) {
     __shared__ int s_w[4];
     const int tid = threadIdx.x;
     const int lane = tid & 63, wv = tid >> 6;
-    const int x0 = blockIdx.x * TS, y0 = blockIdx.y * TS;
-    const unsigned to = a.t_begin + blockIdx.z;
+    const int x0 = bx * TS, y0 = by * TS;
+    const unsigned to = a.t_begin + bz;
     const int qx0 = x0 + (wv & 1) * QS, qy0 = y0 + (wv >> 1) * QS;  // this wave's quadrant
     const int px = qx0 + (lane & (QS - 1)), py = qy0 + lane / QS;
     const bool inside = px < (int)a.W && py < (int)a.H;
@@ -1076,6 +1142,31 @@ __global__ __launch_bounds__(256) void k_aggregate(AArgs a) {
     }
 }
 
+template <int C>
+__global__ __launch_bounds__(256) void k_aggregate(AArgs a) {
+    agg_tile<C>(a, blockIdx.x, blockIdx.y, blockIdx.z);
+}
+
+// Persistent form: one tile per take (the workgroup is the unit); item ->
+// (job, frame, tile row, tile column).
+template <int C>
+__global__ __launch_bounds__(256) void k_aggregate_queue(const AArgs* __restrict__ jobs, WorkQueue q) {
+    __shared__ long s_item;
+    for (;;) {
+        __syncthreads();  // every thread has read the previous item
+        if (threadIdx.x == 0) s_item = (long)atomicAdd(q.counter, 1ull);
+        __syncthreads();
+        const long it = s_item;
+        if (it >= q.total) break;
+        const int j = queue_job(q, it);
+        const AArgs& a = jobs[j];
+        const long local = it - q.item_begin[j];
+        const long tx = (a.W + kAggTile - 1) / kAggTile, ty = (a.H + kAggTile - 1) / kAggTile;
+        const long bz = local / (tx * ty), rem = local - bz * tx * ty;
+        agg_tile<C>(a, (int)(rem % tx), (int)(rem / tx), (unsigned)bz);
+    }
+}
+
 // ---------------------------------------------------------------------
 // Colour transform, 3 channels (src/Utilities/LibVideoT.hpp:777-887).
 // ---------------------------------------------------------------------
@@ -1098,6 +1189,13 @@ __global__ void k_color(float* v, unsigned wh, unsigned F, int forward) {
     r[k] = o0;
     r[wh + k] = o1;
     r[2 * wh + k] = o2;
+}
+
+// Clears the work-queue counters of a batch (a kernel rather than a
+// memset, so that it can sit inside a captured graph).
+__global__ void k_zero_u64(unsigned long long* p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0ull;
 }
 
 // Work counters of one block-matching launch: stats[0] += candidates,
@@ -1225,6 +1323,78 @@ hipError_t launch_adjoint_fallback(float* gz, const float* gy, const float* den,
     hipLaunchKernelGGL(k_adjoint_fallback, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, gz, gy, den, n);
     return hipGetLastError();
 }
+
+// ---- persistent launches over a batch (jobs and queue arrays in device memory)
+template <int K, int C>
+static hipError_t bm_queue_k(const BMArgs* jobs, const WorkQueue& q, int words, int waves, int blocks,
+                             hipStream_t s) {
+    hipLaunchKernelGGL((k_bm_queue<K, C>), dim3(blocks), dim3(64 * waves), (size_t)words * 4 * waves, s, jobs, q,
+                       words);
+    return hipGetLastError();
+}
+
+hipError_t launch_bm_queue(int K, int C, const BMArgs* jobs, const WorkQueue& q, int words, int waves,
+                           int blocks, hipStream_t s) {
+    if (q.total <= 0 || blocks <= 0) return hipSuccess;
+    if ((size_t)words * 4 * waves > 64 * 1024 || waves < 1 || waves > 4) return hipErrorInvalidValue;
+#define VB_BMQ(KK)                                                                               \
+    case KK:                                                                                     \
+        return C == 1 ? bm_queue_k<KK, 1>(jobs, q, words, waves, blocks, s)                      \
+                      : bm_queue_k<KK, 3>(jobs, q, words, waves, blocks, s);
+    switch (K) {
+        VB_BMQ(4) VB_BMQ(5) VB_BMQ(6) VB_BMQ(7) VB_BMQ(8)
+        default: return hipErrorInvalidValue;
+    }
+#undef VB_BMQ
+}
+
+template <int K, int C, int STEP>
+static hipError_t filter_queue_k(const FArgs* jobs, const WorkQueue& q, int words, int waves, int blocks,
+                                 hipStream_t s) {
+    hipLaunchKernelGGL((k_filter_queue<K, C, STEP>), dim3(blocks), dim3(64 * waves), (size_t)words * 4 * waves, s,
+                       jobs, q, words);
+    return hipGetLastError();
+}
+
+template <int K>
+static hipError_t filter_queue_kc(int C, int step, const FArgs* jobs, const WorkQueue& q, int words, int waves,
+                                  int blocks, hipStream_t s) {
+    if (C == 1)
+        return step == 1 ? filter_queue_k<K, 1, 1>(jobs, q, words, waves, blocks, s)
+                         : filter_queue_k<K, 1, 2>(jobs, q, words, waves, blocks, s);
+    return step == 1 ? filter_queue_k<K, 3, 1>(jobs, q, words, waves, blocks, s)
+                     : filter_queue_k<K, 3, 2>(jobs, q, words, waves, blocks, s);
+}
+
+hipError_t launch_filter_queue(int K, int C, int step, const FArgs* jobs, const WorkQueue& q, int words,
+                               int waves, int blocks, hipStream_t s) {
+    if (q.total <= 0 || blocks <= 0) return hipSuccess;
+    if ((size_t)words * 4 * waves > 64 * 1024 || waves < 1 || waves > 4) return hipErrorInvalidValue;
+    switch (K) {
+        case 4: return filter_queue_kc<4>(C, step, jobs, q, words, waves, blocks, s);
+        case 5: return filter_queue_kc<5>(C, step, jobs, q, words, waves, blocks, s);
+        case 6: return filter_queue_kc<6>(C, step, jobs, q, words, waves, blocks, s);
+        case 7: return filter_queue_kc<7>(C, step, jobs, q, words, waves, blocks, s);
+        case 8: return filter_queue_kc<8>(C, step, jobs, q, words, waves, blocks, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_aggregate_queue(int C, const AArgs* jobs, const WorkQueue& q, int blocks, hipStream_t s) {
+    if (q.total <= 0 || blocks <= 0) return hipSuccess;
+    if (C == 1) hipLaunchKernelGGL(k_aggregate_queue<1>, dim3(blocks), dim3(256), 0, s, jobs, q);
+    else hipLaunchKernelGGL(k_aggregate_queue<3>, dim3(blocks), dim3(256), 0, s, jobs, q);
+    return hipGetLastError();
+}
+
+hipError_t launch_zero_u64(unsigned long long* p, int n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_zero_u64, dim3((n + 255) / 256), dim3(256), 0, s, p, n);
+    return hipGetLastError();
+}
+
+int bm_lds_words(int K, int C, int Ns, int Npr, int Nb) { return bm_layout(K, C, Ns, Npr, Nb).total; }
+int filter_words(int K, int C, int N, int step) { return filter_lds_words(K, C, N, step); }
 
 hipError_t launch_aggregate(int C, const AArgs& a, unsigned frames, hipStream_t s) {
     if (frames == 0) return hipSuccess;

@@ -78,6 +78,24 @@ def noisy_clip(width: int, height: int, frames: int, channels: int, sigma: float
     return clean, noisy.astype(np.float32)
 
 
+BATCH_SIZES = ((320, 240), (640, 360), (960, 540), (1280, 720))
+BATCH_SIGMAS = (10.0, 20.0, 30.0, 50.0)
+
+
+def batch_spec(seed: int):
+    """(W, H, F, C, sigma) of the mixed-clip batch's clip `seed`: from
+    default_rng(seed), in this order, F = integers(8, 25), C = 1 or 3
+    (integers(0, 2) -> grey, RGB), the size index integers(0, 4) into
+    BATCH_SIZES and the sigma index integers(0, 4) into BATCH_SIGMAS.  The
+    clip itself is noisy_clip(W, H, F, C, sigma, seed)."""
+    rng = np.random.default_rng(seed)
+    F = int(rng.integers(8, 25))
+    C = 3 if int(rng.integers(0, 2)) else 1
+    W, H = BATCH_SIZES[int(rng.integers(0, 4))]
+    sigma = BATCH_SIGMAS[int(rng.integers(0, 4))]
+    return W, H, F, C, sigma
+
+
 def psnr(ref: np.ndarray, est: np.ndarray) -> float:
     """PSNR as VideoUtils::computePSNR (src/Utilities/LibVideoT.hpp:577-597):
     both clipped to [0, 255], squared float differences summed in double,
