@@ -81,6 +81,7 @@ def main():
     ap.add_argument("--batched", action="store_true",
                     help="instead: the 64-clip mixed batch, batched form against the per-clip path")
     ap.add_argument("--clips", type=int, default=64)
+    ap.add_argument("--no-tune", action="store_true", help="batched: default launch configuration")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -150,7 +151,7 @@ def measure_batched(api, synth, torch, args):
     torch.cuda.synchronize()
     t_clip = (time.perf_counter() - t0) / args.steps
     print(f"[bench] per-clip path {t_clip * 1e3:.1f} ms", file=sys.stderr, flush=True)
-    tuned = api.autotune_batch(items, ctx=ctx)
+    tuned = api.autotune_batch(items, ctx=ctx) if not args.no_tune else None
     print(f"[bench] autotuned {tuned}", file=sys.stderr, flush=True)
     batched()  # plans and captures the graph
     torch.cuda.synchronize()

@@ -251,7 +251,7 @@ struct vbm3d_ctx {
     hipStream_t stream = nullptr;
     DevBuf noisy, basic, den, ff, bf, num_raw, den_raw;  // host-buffer entry points
     DevBuf idx, nsx, nsim, dist, est, wgt, tabs, refxy, meta, gpos, rlist, scal, stats, counts;
-    size_t est_budget = (size_t)64 << 30;
+    size_t est_budget = 0;  // bytes for the per-patch rings; 0 = from free device memory
     // kernel profiling (vbm3d_profile / vbm3d_profile_read)
     bool prof = false;    // bracket launches with events
     bool count = false;   // also accumulate work counters (atomics; not for timed runs)
@@ -305,6 +305,15 @@ hipError_t timed(vbm3d_ctx* ctx, int kernel, int step, hipStream_t s, F&& launch
     return e;
 }
 
+// Bytes the per-patch rings may take: VBM3D_EST_BUDGET_MB when set, else
+// 60 % of the device memory that is free or already held by the rings.
+size_t ring_budget(const vbm3d_ctx* ctx, size_t held) {
+    if (ctx->est_budget) return ctx->est_budget;
+    size_t fr = 0, tot = 0;
+    if (hipMemGetInfo(&fr, &tot) != hipSuccess) return (size_t)32 << 30;
+    return (size_t)((double)(fr + held) * 0.6);
+}
+
 int run_step(vbm3d_ctx* ctx, const StepIO& io, hipStream_t s) {
     const vbm3d_params& p = *io.p;
     int rc = check(p, io.W, io.H, io.F, io.C, io.step);
@@ -332,7 +341,8 @@ int run_step(vbm3d_ctx* ctx, const StepIO& io, hipStream_t s) {
     const unsigned Fr = io.F - p.kt + 1;
     // chunk size from the estimate budget; ring holds S + 2 Nf frames
     const size_t per_frame = (size_t)nrf * C * N * K2 * sizeof(float);
-    size_t cap_frames = std::max<size_t>(1, ctx->est_budget / std::max<size_t>(per_frame, 1));
+    const size_t budget = ring_budget(ctx, ctx->est.cap + ctx->idx.cap + ctx->nsx.cap + ctx->wgt.cap);
+    size_t cap_frames = std::max<size_t>(1, budget / std::max<size_t>(per_frame, 1));
     unsigned S = (unsigned)std::max<long>(1, (long)cap_frames - 2 * (long)p.Nf);
     unsigned ring = std::min<unsigned>(Fr, S + 2 * p.Nf);
     if (S >= Fr) { S = Fr; ring = Fr; }
@@ -625,6 +635,7 @@ int plan_batch(vbm3d_ctx* ctx, const vbm3d_clip* clips, int n, int nb, const vbm
     // shared rings: sub-batches of members whose smallest rings fit the budget,
     // then the largest chunk that fits
     struct Sub { size_t b, e; unsigned S; };
+    const size_t budget = ring_budget(ctx, ctx->b_est.cap + ctx->b_idx.cap + ctx->b_nsx.cap + ctx->b_wgt.cap);
     std::vector<std::vector<Sub>> subs(3);
     size_t need_slots = 0, need_idx = 0, need_est = 0, need_wgt = 0;
     for (int st = 1; st <= 2; st++) {
@@ -635,7 +646,7 @@ int plan_batch(vbm3d_ctx* ctx, const vbm3d_clip* clips, int n, int nb, const vbm
             while (j < ms.size()) {
                 const vbm3d_params& p = *ms[j].p;
                 const size_t r1 = std::min<size_t>(ms[j].Fr, 1 + 2 * p.Nf) * ms[j].per_frame;
-                if (j > i && used + r1 > ctx->est_budget) break;
+                if (j > i && used + r1 > budget) break;
                 used += r1;
                 j++;
             }
@@ -652,7 +663,7 @@ int plan_batch(vbm3d_ctx* ctx, const vbm3d_clip* clips, int n, int nb, const vbm
             unsigned S = cfg.chunk_frames;
             if (S == 0) {
                 S = maxFr;
-                while (S > 1 && bytes_at(S) > ctx->est_budget) S--;
+                while (S > 1 && bytes_at(S) > budget) S--;
             }
             S = std::max(1u, std::min(S, maxFr));
             size_t sb = 0, ib = 0, eb = 0, wb = 0;

@@ -1006,7 +1006,7 @@ __device__ __forceinline__ int lower_bound_u16(const uint16_t* a, int n, int v) 
 // accumulates, for its 8x8 quadrant, the members that overlap the quadrant
 // in list order: every output sample sees its contributions in exactly the
 // reference's order.
-template <int C>
+template <int C, bool RAW>
 __device__ __forceinline__ void agg_tile(const AArgs& a, const int bx, const int by, const unsigned bz) {
     constexpr int TS = kAggTile;
     constexpr int QS = TS / 2;
@@ -1110,10 +1110,14 @@ __device__ __forceinline__ void agg_tile(const AArgs& a, const int bx, const int
                         const float kai = s_kai[kk];
 #pragma unroll
                         for (int c = 0; c < C; c++) {
-                            const float kw = a.raw ? 1.f : kai * a.wgt[(size_t)sl * C + c];
                             const float ev = a.est[(((size_t)sl * C + c) * N + n) * K2 + kk];
-                            pn[c] = pn[c] + (a.raw ? ev : kw * ev);
-                            pd[c] = pd[c] + kw;
+                            if (RAW) {
+                                pn[c] = pn[c] + ev;
+                            } else {
+                                const float kw = kai * a.wgt[(size_t)sl * C + c];
+                                pn[c] = pn[c] + kw * ev;
+                                pd[c] = pd[c] + kw;
+                            }
                         }
                     }
                 }
@@ -1142,9 +1146,9 @@ __device__ __forceinline__ void agg_tile(const AArgs& a, const int bx, const int
     }
 }
 
-template <int C>
+template <int C, bool RAW>
 __global__ __launch_bounds__(256) void k_aggregate(AArgs a) {
-    agg_tile<C>(a, blockIdx.x, blockIdx.y, blockIdx.z);
+    agg_tile<C, RAW>(a, blockIdx.x, blockIdx.y, blockIdx.z);
 }
 
 // Persistent form: one tile per take (the workgroup is the unit); item ->
@@ -1163,7 +1167,7 @@ __global__ __launch_bounds__(256) void k_aggregate_queue(const AArgs* __restrict
         const long local = it - q.item_begin[j];
         const long tx = (a.W + kAggTile - 1) / kAggTile, ty = (a.H + kAggTile - 1) / kAggTile;
         const long bz = local / (tx * ty), rem = local - bz * tx * ty;
-        agg_tile<C>(a, (int)(rem % tx), (int)(rem / tx), (unsigned)bz);
+        agg_tile<C, false>(a, (int)(rem % tx), (int)(rem / tx), (unsigned)bz);
     }
 }
 
@@ -1399,8 +1403,13 @@ int filter_words(int K, int C, int N, int step) { return filter_lds_words(K, C, 
 hipError_t launch_aggregate(int C, const AArgs& a, unsigned frames, hipStream_t s) {
     if (frames == 0) return hipSuccess;
     dim3 grid((a.W + kAggTile - 1) / kAggTile, (a.H + kAggTile - 1) / kAggTile, frames);
-    if (C == 1) hipLaunchKernelGGL(k_aggregate<1>, grid, dim3(256), 0, s, a);
-    else hipLaunchKernelGGL(k_aggregate<3>, grid, dim3(256), 0, s, a);
+    if (a.raw) {
+        if (C == 1) hipLaunchKernelGGL((k_aggregate<1, true>), grid, dim3(256), 0, s, a);
+        else hipLaunchKernelGGL((k_aggregate<3, true>), grid, dim3(256), 0, s, a);
+    } else {
+        if (C == 1) hipLaunchKernelGGL((k_aggregate<1, false>), grid, dim3(256), 0, s, a);
+        else hipLaunchKernelGGL((k_aggregate<3, false>), grid, dim3(256), 0, s, a);
+    }
     return hipGetLastError();
 }
 
