@@ -129,8 +129,8 @@ __device__ void wave_select(const float* ld, const uint32_t* li, int m, int mid,
 // one candidate's sum of squared differences from LDS against the
 // reference patch (also in LDS), in the reference's (c, y, x) order.
 // ---------------------------------------------------------------------
-template <int K, int C>
-__device__ __forceinline__ void bm_one(const BMArgs& a, const long r, float* base, const int lane) {
+template <int K, int C, class A>
+__device__ __forceinline__ void bm_one(const A& a, const long r, float* base, const int lane) {
     const BMLay L = bm_layout(K, C, a.Ns, a.Npr, a.Nb);
     float* tile = base + L.tile;
     float* s_d = base + L.sd;
@@ -439,7 +439,8 @@ __global__ __launch_bounds__(256) void k_bm_queue(const BMArgs* __restrict__ job
         const long it1 = min(q.total, it0 + (long)q.grab);
         for (long it = it0; it < it1; it++) {
             const int j = queue_job(q, it);
-            const BMArgs& a = jobs[j];
+            typedef const __attribute__((address_space(4))) BMArgs cBM;
+            cBM& a = ((cBM*)jobs)[j];  // scalar loads of the job
             bm_one<K, C>(a, a.r_begin + (it - q.item_begin[j]), base, lane);
         }
     }
@@ -592,8 +593,15 @@ __host__ __device__ inline int filter_lds_words(int K, int C, int N, int step) {
     return w;
 }
 
-template <int K, int C, int STEP>
-__device__ __forceinline__ void filter_one(const FArgs& a, const long r, float* X, const int lane) {
+// Channel sigma without a dynamically indexed array (a private array
+// would live in scratch memory).
+template <class A>
+__device__ __forceinline__ float sig_of(const A& a, int c) {
+    return c == 0 ? a.sig[0] : (c == 1 ? a.sig[1] : a.sig[2]);
+}
+
+template <int K, int C, int STEP, class A>
+__device__ __forceinline__ void filter_one(const A& a, const long r, float* X, const int lane) {
     constexpr int K2 = K * K;
     constexpr int PB = 64 / K;           // patches per lane batch
     constexpr int XS = xstride(K), XP = K + 1;
@@ -616,7 +624,7 @@ __device__ __forceinline__ void filter_one(const FArgs& a, const long r, float* 
     const float dct_out = a.tabs->dct_out;
     const int nl = lane / K, rr = lane - nl * K;    // batch patch, row / column index
     const bool lact = lane < PB * K;
-    float weight[C];
+    float wsel = 0.f;  // step 1: lane c ends with channel c's weight
 
     for (int c = 0; c < C; c++) {
         // ---- forward 2D transforms of all members (bior_2d_forward
@@ -665,15 +673,16 @@ __device__ __forceinline__ void filter_one(const FArgs& a, const long r, float* 
         }
         // ---- 1D transform along the group, shrinkage, inverse
         float* v2c = v2 + c * K2 * a.N;
+        const float sgc = sig_of(a, c);
         float wc;
         switch (nsx) {
-            case 1: wc = shrink<K, 1, STEP>(zg, eg, v2c, lane, had, a.lambda, a.sig[c]); break;
-            case 2: wc = shrink<K, 2, STEP>(zg, eg, v2c, lane, had, a.lambda, a.sig[c]); break;
-            case 4: wc = shrink<K, 4, STEP>(zg, eg, v2c, lane, had, a.lambda, a.sig[c]); break;
-            case 8: wc = shrink<K, 8, STEP>(zg, eg, v2c, lane, had, a.lambda, a.sig[c]); break;
-            default: wc = shrink<K, 16, STEP>(zg, eg, v2c, lane, had, a.lambda, a.sig[c]); break;
+            case 1: wc = shrink<K, 1, STEP>(zg, eg, v2c, lane, had, a.lambda, sgc); break;
+            case 2: wc = shrink<K, 2, STEP>(zg, eg, v2c, lane, had, a.lambda, sgc); break;
+            case 4: wc = shrink<K, 4, STEP>(zg, eg, v2c, lane, had, a.lambda, sgc); break;
+            case 8: wc = shrink<K, 8, STEP>(zg, eg, v2c, lane, had, a.lambda, sgc); break;
+            default: wc = shrink<K, 16, STEP>(zg, eg, v2c, lane, had, a.lambda, sgc); break;
         }
-        weight[c] = wc;
+        if (lane == c) wsel = wc;
         wave_sync();
         // ---- inverse 2D transforms (bior_2d_inverse src/lib_transforms.cpp:
         //      270-321; dct_2d_inv src/vbm3d.cpp:1678-1713), store [n][pixel]
@@ -721,17 +730,13 @@ __device__ __forceinline__ void filter_one(const FArgs& a, const long r, float* 
             const float* vc = v2 + lane * K2 * a.N;
             float acc = 0.f;
             for (int f = 0; f < K2 * nsx; f++) acc = acc + vc[f];
-            const float sg = a.sig[lane], s2 = sg * sg;
+            const float sg = sig_of(a, lane), s2 = sg * sg;
             float wgt;
             if (had) wgt = acc > 0.0f ? 1.0f / (s2 * acc) : 1.0f;
             else wgt = 1.f / (s2 * acc);
             a.wgt[(size_t)slot * C + lane] = wgt;
         }
     } else if (lane < C) {
-        float wsel = weight[0];
-#pragma unroll
-        for (int c = 1; c < C; c++)
-            if (lane == c) wsel = weight[c];
         a.wgt[(size_t)slot * C + lane] = wsel;
     }
 }
@@ -758,7 +763,8 @@ __global__ __launch_bounds__(256) void k_filter_queue(const FArgs* __restrict__ 
         const long it1 = min(q.total, it0 + (long)q.grab);
         for (long it = it0; it < it1; it++) {
             const int j = queue_job(q, it);
-            const FArgs& a = jobs[j];
+            typedef const __attribute__((address_space(4))) FArgs cFA;
+            cFA& a = ((cFA*)jobs)[j];  // scalar loads of the job
             filter_one<K, C, STEP>(a, a.r_begin + (it - q.item_begin[j]), X, lane);
         }
     }
@@ -1006,8 +1012,8 @@ __device__ __forceinline__ int lower_bound_u16(const uint16_t* a, int n, int v) 
 // accumulates, for its 8x8 quadrant, the members that overlap the quadrant
 // in list order: every output sample sees its contributions in exactly the
 // reference's order.
-template <int C, bool RAW>
-__device__ __forceinline__ void agg_tile(const AArgs& a, const int bx, const int by, const unsigned bz) {
+template <int C, bool RAW, class A>
+__device__ __forceinline__ void agg_tile(const A& a, const int bx, const int by, const unsigned bz) {
     constexpr int TS = kAggTile;
     constexpr int QS = TS / 2;
     __shared__ uint32_t L_sn[256 * kMaxN];
@@ -1033,6 +1039,7 @@ __device__ __forceinline__ void agg_tile(const AArgs& a, const int bx, const int
     // a member at q = to*whc + y*W + x overlaps the tile's rows iff
     // q - to*whc lies in [ylo*W, yhi*W)
     const unsigned tbase = to * whc;
+    const Div32 dW = {a.divW.m, a.divW.s1, a.divW.s2};
     const unsigned rmin = (unsigned)max(0, y0 - K + 1) * W, rmax = (unsigned)min((int)a.H, y0 + TS) * W;
 
     for (int ci = 0; ci < a.ncrop; ci++) {
@@ -1062,7 +1069,7 @@ __device__ __forceinline__ void agg_tile(const AArgs& a, const int bx, const int
                 for (int n = 0; n < ns; n++) {
                     const unsigned r = mem[n] - tbase;
                     if (r < rmin || r >= rmax) continue;
-                    const int qy = (int)div32(r, a.divW), qx = (int)(r - (unsigned)qy * W);
+                    const int qy = (int)div32(r, dW), qx = (int)(r - (unsigned)qy * W);
                     if (qx <= x0 + TS - 1 && qx + K - 1 >= x0) hits |= 1u << n;
                 }
             }
@@ -1083,7 +1090,7 @@ __device__ __forceinline__ void agg_tile(const AArgs& a, const int bx, const int
             for (int n = 0; n < N; n++)
                 if ((hits >> n) & 1u) {
                     const unsigned r = a.idx[(size_t)slot * N + n] - tbase;
-                    const unsigned qy = div32(r, a.divW), qx = r - qy * W;
+                    const unsigned qy = div32(r, dW), qx = r - qy * W;
                     L_sn[o] = slot * kMaxN + n;
                     L_pos[o] = qx | (qy << 16);
                     o++;
@@ -1163,7 +1170,8 @@ __global__ __launch_bounds__(256) void k_aggregate_queue(const AArgs* __restrict
         const long it = s_item;
         if (it >= q.total) break;
         const int j = queue_job(q, it);
-        const AArgs& a = jobs[j];
+        typedef const __attribute__((address_space(4))) AArgs cAA;
+        cAA& a = ((cAA*)jobs)[j];  // scalar loads of the job
         const long local = it - q.item_begin[j];
         const long tx = (a.W + kAggTile - 1) / kAggTile, ty = (a.H + kAggTile - 1) / kAggTile;
         const long bz = local / (tx * ty), rem = local - bz * tx * ty;
