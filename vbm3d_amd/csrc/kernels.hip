@@ -35,13 +35,28 @@ __device__ __forceinline__ void wave_sync() {
 // A global work queue: items [0, total) of a batched launch, taken `grab`
 // at a time through an atomic counter (zeroed before the launch); job j
 // owns items [item_begin[j], item_begin[j+1]).
-__device__ __forceinline__ long queue_take(const WorkQueue& q, int lane) {
+// queue_issue sends the atomic (lane 0 holds the reply); queue_value makes
+// it wave-uniform.  Issuing the next take before working on the current
+// items hides the atomic's round trip.
+__device__ __forceinline__ unsigned long long queue_issue(const WorkQueue& q, int lane) {
     unsigned long long v = 0;
     if (lane == 0) v = atomicAdd(q.counter, (unsigned long long)q.grab);
+    return v;
+}
+
+__device__ __forceinline__ long queue_value(unsigned long long v) {
     const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
     const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
     return (long)(((unsigned long long)hi << 32) | lo);
 }
+
+__device__ __forceinline__ long queue_take(const WorkQueue& q, int lane) {
+    return queue_value(queue_issue(q, lane));
+}
+
+// The job that owns item `it`, starting from the cached job j with items
+// [jb, je): a search only when `it` has left that range.
+__device__ __forceinline__ void queue_seek(const WorkQueue& q, long it, int& j, long& jb, long& je);
 
 __device__ __forceinline__ int queue_job(const WorkQueue& q, long it) {
     int lo = 0, hi = q.njobs - 1;  // largest j with item_begin[j] <= it
@@ -51,6 +66,13 @@ __device__ __forceinline__ int queue_job(const WorkQueue& q, long it) {
         else hi = mid - 1;
     }
     return lo;
+}
+
+__device__ __forceinline__ void queue_seek(const WorkQueue& q, long it, int& j, long& jb, long& je) {
+    if (it >= jb && it < je) return;
+    j = queue_job(q, it);
+    jb = q.item_begin[j];
+    je = q.item_begin[j + 1];
 }
 
 // Monotone map float -> uint32 (for finite inputs); -0 and +0 map alike so
@@ -433,16 +455,19 @@ __global__ __launch_bounds__(256) void k_bm_queue(const BMArgs* __restrict__ job
     extern __shared__ float smem[];
     const int lane = threadIdx.x & 63;
     float* base = smem + (size_t)(threadIdx.x >> 6) * words;
-    for (;;) {
-        const long it0 = queue_take(q, lane);
-        if (it0 >= q.total) break;
+    typedef const __attribute__((address_space(4))) BMArgs cBM;
+    int j = 0;
+    long jb = 0, je = 0;
+    long it0 = queue_take(q, lane);
+    while (it0 < q.total) {
+        const unsigned long long nxt = queue_issue(q, lane);
         const long it1 = min(q.total, it0 + (long)q.grab);
         for (long it = it0; it < it1; it++) {
-            const int j = queue_job(q, it);
-            typedef const __attribute__((address_space(4))) BMArgs cBM;
+            queue_seek(q, it, j, jb, je);
             cBM& a = ((cBM*)jobs)[j];  // scalar loads of the job
-            bm_one<K, C>(a, a.r_begin + (it - q.item_begin[j]), base, lane);
+            bm_one<K, C>(a, a.r_begin + (it - jb), base, lane);
         }
+        it0 = queue_value(nxt);
     }
 }
 
@@ -757,16 +782,19 @@ __global__ __launch_bounds__(256) void k_filter_queue(const FArgs* __restrict__ 
     extern __shared__ float smem[];
     const int lane = threadIdx.x & 63;
     float* X = smem + (size_t)(threadIdx.x >> 6) * words;
-    for (;;) {
-        const long it0 = queue_take(q, lane);
-        if (it0 >= q.total) break;
+    typedef const __attribute__((address_space(4))) FArgs cFA;
+    int j = 0;
+    long jb = 0, je = 0;
+    long it0 = queue_take(q, lane);
+    while (it0 < q.total) {
+        const unsigned long long nxt = queue_issue(q, lane);
         const long it1 = min(q.total, it0 + (long)q.grab);
         for (long it = it0; it < it1; it++) {
-            const int j = queue_job(q, it);
-            typedef const __attribute__((address_space(4))) FArgs cFA;
+            queue_seek(q, it, j, jb, je);
             cFA& a = ((cFA*)jobs)[j];  // scalar loads of the job
-            filter_one<K, C, STEP>(a, a.r_begin + (it - q.item_begin[j]), X, lane);
+            filter_one<K, C, STEP>(a, a.r_begin + (it - jb), X, lane);
         }
+        it0 = queue_value(nxt);
     }
 }
 
