@@ -161,12 +161,15 @@ typedef struct vbm3d_batch_config {
     int grab;               /* reference patches a wave takes from the work queue at a time, >= 1 */
     int groups_per_cu;      /* persistent workgroups per compute unit, >= 1 */
     int use_graph;          /* capture the whole batch in a HIP graph and replay it on later identical calls */
+    int persistent;         /* 1: matching and filtering run as persistent workgroups that take patches from
+                               the global work queue; 0: one wave per patch, in workgroup dispatch order */
 } vbm3d_batch_config;
 
 /* Denoises `nclips` clips of any sizes and parameters together: per step,
- * per chunk of frames and per kernel class, one persistent launch whose
- * workgroups take reference patches (or output tiles) of all clips from a
- * global work queue.  With use_graph the whole sequence is captured once
+ * per chunk of frames and per kernel class, one launch over the
+ * reference patches (or output tiles) of all clips, ordered as a global
+ * work queue; with `persistent`, resident workgroups take them from the
+ * queue through an atomic counter.  With use_graph the whole sequence is captured once
  * in a HIP graph and replayed on later calls with the same clips,
  * buffers and configuration (not when a clip has flows).  cfg NULL means
  * the configuration vbm3d_autotune_batch last chose for this context, or
@@ -178,7 +181,8 @@ int vbm3d_run_batch(vbm3d_ctx* ctx, const vbm3d_clip* clips, int nclips,
                     void* stream);
 
 /* Times candidate configurations of the batched form on these clips
- * (coordinate search over waves_per_group, grab and groups_per_cu) and
+ * (coordinate search over persistent, waves_per_group, grab and
+ * groups_per_cu) and
  * returns the fastest in *best; the context keeps it for later
  * vbm3d_run_batch calls with cfg NULL.  The clips' buffers are restored
  * after each trial, so inputs are left as they were. */

@@ -60,10 +60,11 @@ def reference(env):
 
 
 @pytest.mark.parametrize("cfg", [
-    dict(chunk_frames=0, waves_per_group=0, grab=1, groups_per_cu=8, use_graph=0),
-    dict(chunk_frames=1, waves_per_group=1, grab=4, groups_per_cu=2, use_graph=0),
-    dict(chunk_frames=2, waves_per_group=2, grab=2, groups_per_cu=16, use_graph=0),
-    dict(chunk_frames=3, waves_per_group=4, grab=8, groups_per_cu=1, use_graph=0),
+    dict(chunk_frames=0, waves_per_group=0, grab=1, groups_per_cu=8, use_graph=0, persistent=0),
+    dict(chunk_frames=1, waves_per_group=1, grab=4, groups_per_cu=2, use_graph=0, persistent=1),
+    dict(chunk_frames=2, waves_per_group=2, grab=2, groups_per_cu=16, use_graph=0, persistent=1),
+    dict(chunk_frames=3, waves_per_group=4, grab=8, groups_per_cu=1, use_graph=0, persistent=1),
+    dict(chunk_frames=1, waves_per_group=2, grab=1, groups_per_cu=1, use_graph=0, persistent=0),
 ])
 def test_batch_matches_per_clip(env, reference, cfg):
     torch, V, dev, clips = env
@@ -77,7 +78,7 @@ def test_batch_graph_replays(env, reference):
     between the replays; every replay equals the per-clip result."""
     torch, V, dev, clips = env
     items = _items(env)
-    cfg = dict(chunk_frames=2, waves_per_group=0, grab=2, groups_per_cu=4, use_graph=1)
+    cfg = dict(chunk_frames=2, waves_per_group=0, grab=2, groups_per_cu=4, use_graph=1, persistent=1)
     for rep in range(4):
         _restore(env, items)
         for it in items:
@@ -94,7 +95,8 @@ def test_batch_threaded_grid(env):
     torch, V, dev, clips = env
     ref = _per_clip(env, nb=4)
     items = _items(env)
-    V.run_batch(items, 4, dict(chunk_frames=0, waves_per_group=0, grab=1, groups_per_cu=8, use_graph=0))
+    V.run_batch(items, 4, dict(chunk_frames=0, waves_per_group=0, grab=1, groups_per_cu=8, use_graph=0,
+                               persistent=0))
     _same(items, ref, "nb=4")
 
 

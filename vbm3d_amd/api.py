@@ -58,7 +58,7 @@ class Clip(C.Structure):
 class BatchConfig(C.Structure):
     """POD mirror of vbm3d_batch_config."""
     _fields_ = [("chunk_frames", C.c_uint), ("waves_per_group", C.c_int), ("grab", C.c_int),
-                ("groups_per_cu", C.c_int), ("use_graph", C.c_int)]
+                ("groups_per_cu", C.c_int), ("use_graph", C.c_int), ("persistent", C.c_int)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}

@@ -890,11 +890,18 @@ int exec_batch(vbm3d_ctx* ctx, const BatchPlan& plan, const vbm3d_batch_config& 
             if ((size_t)op.words * 4 > 64 * 1024) return fail(VBM3D_ERR_UNSUPPORTED, "patch working set above 64 KB of LDS");
             const long need = (op.total + (long)q.grab * waves - 1) / ((long)q.grab * waves);
             const long blocks = std::min<long>(need, (long)cfg.groups_per_cu * cus);
-            if (op.kind == 0)
+            if (!cfg.persistent) {
+                if (op.kind == 0)
+                    HIP_TRY(launch_bm_multi(op.K, op.C, (const BMArgs*)(blob + op.jobs), q, op.words, waves, s));
+                else
+                    HIP_TRY(launch_filter_multi(op.K, op.C, op.step, (const FArgs*)(blob + op.jobs), q, op.words,
+                                                waves, s));
+            } else if (op.kind == 0) {
                 HIP_TRY(launch_bm_queue(op.K, op.C, (const BMArgs*)(blob + op.jobs), q, op.words, waves, (int)blocks, s));
-            else
+            } else {
                 HIP_TRY(launch_filter_queue(op.K, op.C, op.step, (const FArgs*)(blob + op.jobs), q, op.words, waves,
                                             (int)blocks, s));
+            }
         } else if (op.kind == 3) {
             HIP_TRY(launch_color(op.v, op.W, op.H, op.F, op.fwd != 0, s));
         } else {
@@ -911,6 +918,7 @@ vbm3d_batch_config default_batch_config() {
     c.grab = 1;
     c.groups_per_cu = 8;
     c.use_graph = 1;
+    c.persistent = 0;
     return c;
 }
 
@@ -918,6 +926,7 @@ int check_batch_config(const vbm3d_batch_config& c) {
     if (c.waves_per_group < 0 || c.waves_per_group > 4) return fail(VBM3D_ERR_ARG, "waves_per_group must be in [0, 4]");
     if (c.grab < 1 || c.grab > 64) return fail(VBM3D_ERR_ARG, "grab must be in [1, 64]");
     if (c.groups_per_cu < 1 || c.groups_per_cu > 64) return fail(VBM3D_ERR_ARG, "groups_per_cu must be in [1, 64]");
+    if (c.persistent != 0 && c.persistent != 1) return fail(VBM3D_ERR_ARG, "persistent must be 0 or 1");
     return VBM3D_OK;
 }
 
@@ -1358,17 +1367,22 @@ int vbm3d_autotune_batch(vbm3d_ctx* ctx, const vbm3d_clip* clips, int nclips, in
     float best_ms = 0;
     rc = trial(cur, &best_ms);  // also warms up
     if (!rc) rc = trial(cur, &best_ms);
+    // coordinate search: dispatch mode, waves per group, then (persistent
+    // only) the take size and the resident groups per CU
+    const int mode_c[] = {0, 1};
     const int waves_c[] = {1, 2, 4};
-    const int grab_c[] = {1, 2, 4, 8};
+    const int grab_c[] = {1, 4, 16};
     const int gpc_c[] = {2, 4, 8, 16};
-    for (int dim = 0; dim < 3 && !rc; dim++) {
-        const int* vals = dim == 0 ? waves_c : dim == 1 ? grab_c : gpc_c;
-        const int nv = dim == 0 ? 3 : 4;
+    for (int dim = 0; dim < 4 && !rc; dim++) {
+        if (dim >= 2 && !cur.persistent) break;
+        const int* vals = dim == 0 ? mode_c : dim == 1 ? waves_c : dim == 2 ? grab_c : gpc_c;
+        const int nv = dim == 0 ? 2 : dim == 1 ? 3 : dim == 2 ? 3 : 4;
         vbm3d_batch_config keep = cur;
         for (int v = 0; v < nv && !rc; v++) {
             vbm3d_batch_config c = keep;
-            if (dim == 0) c.waves_per_group = vals[v];
-            else if (dim == 1) c.grab = vals[v];
+            if (dim == 0) c.persistent = vals[v];
+            else if (dim == 1) c.waves_per_group = vals[v];
+            else if (dim == 2) c.grab = vals[v];
             else c.groups_per_cu = vals[v];
             float ms = 0;
             rc = trial(c, &ms);

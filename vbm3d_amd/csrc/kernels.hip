@@ -50,6 +50,12 @@ __device__ __forceinline__ long queue_value(unsigned long long v) {
     return (long)(((unsigned long long)hi << 32) | lo);
 }
 
+__device__ __forceinline__ long uniform_long(long v) {
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)((unsigned long long)v >> 32));
+    return (long)(((unsigned long long)hi << 32) | lo);
+}
+
 __device__ __forceinline__ long queue_take(const WorkQueue& q, int lane) {
     return queue_value(queue_issue(q, lane));
 }
@@ -65,14 +71,14 @@ __device__ __forceinline__ int queue_job(const WorkQueue& q, long it) {
         if (q.item_begin[mid] <= it) lo = mid;
         else hi = mid - 1;
     }
-    return lo;
+    return __builtin_amdgcn_readfirstlane(lo);  // wave-uniform: scalar loads of the job follow
 }
 
 __device__ __forceinline__ void queue_seek(const WorkQueue& q, long it, int& j, long& jb, long& je) {
     if (it >= jb && it < je) return;
     j = queue_job(q, it);
-    jb = q.item_begin[j];
-    je = q.item_begin[j + 1];
+    jb = uniform_long(q.item_begin[j]);
+    je = uniform_long(q.item_begin[j + 1]);
 }
 
 // Monotone map float -> uint32 (for finite inputs); -0 and +0 map alike so
@@ -448,6 +454,18 @@ __global__ __launch_bounds__(256) void k_bm(BMArgs a) {
     bm_one<K, C>(a, r, smem + (size_t)w * L.total, lane);
 }
 
+template <int K, int C>
+__global__ __launch_bounds__(256) void k_bm_multi(const BMArgs* __restrict__ jobs, WorkQueue q, int words) {
+    extern __shared__ float smem[];
+    const int lane = threadIdx.x & 63;
+    const long g = uniform_long((long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
+    if (g >= q.total) return;
+    const int j = queue_job(q, g);
+    typedef const __attribute__((address_space(4))) BMArgs cBM;
+    cBM& a = ((cBM*)jobs)[j];
+    bm_one<K, C>(a, a.r_begin + (g - uniform_long(q.item_begin[j])), smem + (size_t)(threadIdx.x >> 6) * words, lane);
+}
+
 // Persistent form over a batch: every wave takes `grab` items at a time
 // from the queue counter until the queue is empty; item -> (job, patch).
 template <int K, int C>
@@ -775,6 +793,21 @@ __global__ __launch_bounds__(256) void k_filter(FArgs a) {
     const long r = a.r_begin + (long)blockIdx.x * nw + w;
     if (r >= a.r_end) return;
     filter_one<K, C, STEP>(a, r, smem + (size_t)w * filter_lds_words(K, C, a.N, STEP), lane);
+}
+
+// One item per wave over a batch: global wave g takes item g, so the
+// hardware's workgroup dispatch order is the queue.
+template <int K, int C, int STEP>
+__global__ __launch_bounds__(256) void k_filter_multi(const FArgs* __restrict__ jobs, WorkQueue q, int words) {
+    extern __shared__ float smem[];
+    const int lane = threadIdx.x & 63;
+    const long g = uniform_long((long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
+    if (g >= q.total) return;
+    const int j = queue_job(q, g);
+    typedef const __attribute__((address_space(4))) FArgs cFA;
+    cFA& a = ((cFA*)jobs)[j];
+    filter_one<K, C, STEP>(a, a.r_begin + (g - uniform_long(q.item_begin[j])),
+                           smem + (size_t)(threadIdx.x >> 6) * words, lane);
 }
 
 template <int K, int C, int STEP>
@@ -1373,6 +1406,28 @@ static hipError_t bm_queue_k(const BMArgs* jobs, const WorkQueue& q, int words, 
     return hipGetLastError();
 }
 
+template <int K, int C>
+static hipError_t bm_multi_k(const BMArgs* jobs, const WorkQueue& q, int words, int waves, hipStream_t s) {
+    const long blocks = (q.total + waves - 1) / waves;
+    hipLaunchKernelGGL((k_bm_multi<K, C>), dim3((unsigned)blocks), dim3(64 * waves), (size_t)words * 4 * waves, s,
+                       jobs, q, words);
+    return hipGetLastError();
+}
+
+hipError_t launch_bm_multi(int K, int C, const BMArgs* jobs, const WorkQueue& q, int words, int waves,
+                           hipStream_t s) {
+    if (q.total <= 0) return hipSuccess;
+    if ((size_t)words * 4 * waves > 64 * 1024 || waves < 1 || waves > 4) return hipErrorInvalidValue;
+#define VB_BMM(KK)                                                                               \
+    case KK:                                                                                     \
+        return C == 1 ? bm_multi_k<KK, 1>(jobs, q, words, waves, s) : bm_multi_k<KK, 3>(jobs, q, words, waves, s);
+    switch (K) {
+        VB_BMM(4) VB_BMM(5) VB_BMM(6) VB_BMM(7) VB_BMM(8)
+        default: return hipErrorInvalidValue;
+    }
+#undef VB_BMM
+}
+
 hipError_t launch_bm_queue(int K, int C, const BMArgs* jobs, const WorkQueue& q, int words, int waves,
                            int blocks, hipStream_t s) {
     if (q.total <= 0 || blocks <= 0) return hipSuccess;
@@ -1404,6 +1459,38 @@ static hipError_t filter_queue_kc(int C, int step, const FArgs* jobs, const Work
                          : filter_queue_k<K, 1, 2>(jobs, q, words, waves, blocks, s);
     return step == 1 ? filter_queue_k<K, 3, 1>(jobs, q, words, waves, blocks, s)
                      : filter_queue_k<K, 3, 2>(jobs, q, words, waves, blocks, s);
+}
+
+template <int K, int C, int STEP>
+static hipError_t filter_multi_k(const FArgs* jobs, const WorkQueue& q, int words, int waves, hipStream_t s) {
+    const long blocks = (q.total + waves - 1) / waves;
+    hipLaunchKernelGGL((k_filter_multi<K, C, STEP>), dim3((unsigned)blocks), dim3(64 * waves),
+                       (size_t)words * 4 * waves, s, jobs, q, words);
+    return hipGetLastError();
+}
+
+template <int K>
+static hipError_t filter_multi_kc(int C, int step, const FArgs* jobs, const WorkQueue& q, int words, int waves,
+                                  hipStream_t s) {
+    if (C == 1)
+        return step == 1 ? filter_multi_k<K, 1, 1>(jobs, q, words, waves, s)
+                         : filter_multi_k<K, 1, 2>(jobs, q, words, waves, s);
+    return step == 1 ? filter_multi_k<K, 3, 1>(jobs, q, words, waves, s)
+                     : filter_multi_k<K, 3, 2>(jobs, q, words, waves, s);
+}
+
+hipError_t launch_filter_multi(int K, int C, int step, const FArgs* jobs, const WorkQueue& q, int words,
+                               int waves, hipStream_t s) {
+    if (q.total <= 0) return hipSuccess;
+    if ((size_t)words * 4 * waves > 64 * 1024 || waves < 1 || waves > 4) return hipErrorInvalidValue;
+    switch (K) {
+        case 4: return filter_multi_kc<4>(C, step, jobs, q, words, waves, s);
+        case 5: return filter_multi_kc<5>(C, step, jobs, q, words, waves, s);
+        case 6: return filter_multi_kc<6>(C, step, jobs, q, words, waves, s);
+        case 7: return filter_multi_kc<7>(C, step, jobs, q, words, waves, s);
+        case 8: return filter_multi_kc<8>(C, step, jobs, q, words, waves, s);
+        default: return hipErrorInvalidValue;
+    }
 }
 
 hipError_t launch_filter_queue(int K, int C, int step, const FArgs* jobs, const WorkQueue& q, int words,
