@@ -223,7 +223,8 @@ int check(const vbm3d_params& p, unsigned W, unsigned H, unsigned F, unsigned C,
     if (p.Nb == 0 || p.Nb > 8) return no(VBM3D_ERR_UNSUPPORTED, "Nb must be in [1, 8]");
     if (p.Ns == 0 || p.Ns > (unsigned)kMaxWin || p.Npr == 0 || p.Npr > (unsigned)kMaxWin)
         return no(VBM3D_ERR_UNSUPPORTED, "Ns and Npr must be in [1, 8]");
-    if (2 * p.Nb * p.Npr * p.Npr > (unsigned)kMaxSlots) return no(VBM3D_ERR_UNSUPPORTED, "2 Nb Npr^2 > 512");
+    if (2 * p.Nb * (unsigned)pow2ceil((int)(p.Npr * p.Npr)) > (unsigned)kMaxSlots)
+        return no(VBM3D_ERR_UNSUPPORTED, "2 Nb Npr^2 (Npr^2 rounded up to a power of two) > 512");
     if (p.Nb * p.Nb > 64) return no(VBM3D_ERR_UNSUPPORTED, "Nb^2 > 64");
     if (p.Nb * (2 * p.Nf + 1) > (unsigned)kMaxBest) return no(VBM3D_ERR_UNSUPPORTED, "Nb * (2 Nf + 1) > 64");
     if (p.p == 0) return no(VBM3D_ERR_ARG, "step p must be > 0");
@@ -379,6 +380,8 @@ int run_step(vbm3d_ctx* ctx, const StepIO& io, hipStream_t s) {
     bm.ring = ring;
     bm.Ns = p.Ns; bm.Npr = p.Npr; bm.Nb = p.Nb; bm.Nf = p.Nf; bm.N = p.N; bm.kt = p.kt;
     bm.d = p.d; bm.tau = p.tau;
+    bm.dW = make_div32(io.W);
+    bm.dWHC = make_div32(bm.whc);
     bm.idx_out = (uint32_t*)ctx->idx.p;
     bm.dist_out = nullptr;
     bm.nsim_out = nullptr;
@@ -403,6 +406,8 @@ int run_step(vbm3d_ctx* ctx, const StepIO& io, hipStream_t s) {
     fa.wgt = (float*)ctx->wgt.p;
     fa.gy = io.gy;
     fa.den = io.den;
+    fa.dW = bm.dW;
+    fa.dWHC = bm.dWHC;
 
     AArgs ag{};
     ag.W = io.W; ag.H = io.H; ag.F = io.F; ag.C = C; ag.wh = bm.wh; ag.whc = bm.whc;
@@ -752,6 +757,8 @@ int plan_batch(vbm3d_ctx* ctx, const vbm3d_clip* clips, int n, int nb, const vbm
                         bm.ring = m.ring;
                         bm.Ns = p.Ns; bm.Npr = p.Npr; bm.Nb = p.Nb; bm.Nf = p.Nf; bm.N = p.N; bm.kt = p.kt;
                         bm.d = p.d; bm.tau = p.tau;
+                        bm.dW = make_div32(c.W);
+                        bm.dWHC = make_div32(whc);
                         bm.idx_out = d_idx + m.idx_base;
                         bm.dist_out = nullptr;
                         bm.nsim_out = nullptr;
@@ -770,6 +777,7 @@ int plan_batch(vbm3d_ctx* ctx, const vbm3d_clip* clips, int n, int nb, const vbm
                         fa.est = d_est + m.est_base;
                         fa.wgt = d_wgt + m.wgt_base;
                         fa.gy = nullptr; fa.den = nullptr;
+                        fa.dW = bm.dW; fa.dWHC = bm.dWHC;
                         Cls* cl = nullptr;
                         for (Cls& x : cls)
                             if (x.K == (int)m.K && x.C == (int)m.C) cl = &x;
@@ -1189,6 +1197,8 @@ int vbm3d_block_match(vbm3d_ctx* ctx, const float* vid, unsigned W, unsigned H, 
     bm.nrf = 1; bm.ring = 1;
     bm.Ns = prm->Ns; bm.Npr = prm->Npr; bm.Nb = prm->Nb; bm.Nf = prm->Nf; bm.N = prm->N; bm.kt = prm->kt;
     bm.d = prm->d; bm.tau = prm->tau;
+    bm.dW = make_div32(W);
+    bm.dWHC = make_div32(W * H * C);
     bm.idx_out = (uint32_t*)ctx->idx.p;
     bm.dist_out = (float*)ctx->dist.p;
     bm.nsim_out = (uint32_t*)ctx->nsim.p;

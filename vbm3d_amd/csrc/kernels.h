@@ -12,6 +12,33 @@ constexpr int kMaxBest = 64;    // Nb * (2 Nf + 1) bound
 constexpr int kMaxSlots = 512;  // Nb * Npr^2 bound
 constexpr int kAggTile = 16;    // aggregation tile edge (pixels)
 
+// Exact unsigned 32-bit division by a run-time constant d >= 1 with a
+// multiply-high and two shifts (Granlund and Montgomery, "Division by
+// invariant integers using multiplication", PLDI 1994, round-up variant):
+// l = ceil(log2 d), m = floor(2^32 (2^l - d) / d) + 1,
+// n / d = (t + ((n - t) >> min(l, 1))) >> max(l - 1, 0), t = mulhi(m, n).
+struct Div32 {
+    uint32_t m, s1, s2;
+};
+
+__host__ __device__ inline Div32 make_div32(uint32_t d) {
+    uint32_t l = 0;
+    while (((uint64_t)1 << l) < d) l++;
+    const uint64_t m = (((((uint64_t)1) << l) - d) << 32) / d + 1;
+    return Div32{(uint32_t)m, l ? 1u : 0u, l ? l - 1 : 0u};
+}
+
+__host__ __device__ inline uint32_t div32(uint32_t n, Div32 v) {
+    const uint32_t t = (uint32_t)(((uint64_t)v.m * n) >> 32);
+    return (t + ((n - t) >> v.s1)) >> v.s2;
+}
+
+__host__ __device__ inline int pow2ceil(int v) {
+    int r = 1;
+    while (r < v) r *= 2;
+    return r;
+}
+
 // Transform tables of one step, uploaded once per step.
 struct Tabs {
     float biorF[64];   // analysis rows [j][s]
@@ -38,7 +65,9 @@ __host__ __device__ inline BMLay bm_layout(int K, int C, int Ns, int Npr, int Nb
     L.tp0 = L.th0 | 1;
     L.th1 = Npr + K - 1;
     L.tp1 = L.th1 | 1;
-    const int S = ((2 * Nb * Npr * Npr + 127) / 128) * 128;
+    // predictive-stage slots: window w owns [w * P2p, (w + 1) * P2p),
+    // P2p = Npr^2 rounded up to a power of two
+    const int S = ((2 * Nb * pow2ceil(Npr * Npr) + 127) / 128) * 128;
     const int t0 = L.th0 * L.tp0 * C, t1 = 2 * Nb * L.th1 * L.tp1 * C;
     int t = t0 > t1 ? t0 : t1;
     t = t > 128 ? t : 128;  // the slow-path scratch (64 entries) reuses the tile
@@ -78,6 +107,7 @@ struct BMArgs {
     float* dist_out;     // [slot][N] or null
     uint32_t* nsim_out;  // [slot] or null
     uint32_t* nsx_out;   // [slot]
+    Div32 dW, dWHC;       // division by W and by W*H*C
     uint32_t* count_out;  // null, or per patch of the launch, at 2*(r - r_begin):
                           // candidates evaluated, group size
 };
@@ -96,30 +126,10 @@ struct FArgs {
     const Tabs* tabs;
     float* est;  // [slot][C][N][K*K]
     float* wgt;  // [slot][C]
+    Div32 dW, dWHC;      // division by W and by W*H*C
     const float* gy;   // step-2 adjoint: gradient with respect to the output
     const float* den;  // step-2 adjoint: the forward pass's denominator
 };
-
-// Exact unsigned 32-bit division by a run-time constant d >= 1 with a
-// multiply-high and two shifts (Granlund and Montgomery, "Division by
-// invariant integers using multiplication", PLDI 1994, round-up variant):
-// l = ceil(log2 d), m = floor(2^32 (2^l - d) / d) + 1,
-// n / d = (t + ((n - t) >> min(l, 1))) >> max(l - 1, 0), t = mulhi(m, n).
-struct Div32 {
-    uint32_t m, s1, s2;
-};
-
-__host__ __device__ inline Div32 make_div32(uint32_t d) {
-    uint32_t l = 0;
-    while (((uint64_t)1 << l) < d) l++;
-    const uint64_t m = (((((uint64_t)1) << l) - d) << 32) / d + 1;
-    return Div32{(uint32_t)m, l ? 1u : 0u, l ? l - 1 : 0u};
-}
-
-__host__ __device__ inline uint32_t div32(uint32_t n, Div32 v) {
-    const uint32_t t = (uint32_t)(((uint64_t)v.m * n) >> 32);
-    return (t + ((n - t) >> v.s1)) >> v.s2;
-}
 
 struct AArgs {
     unsigned W, H, F, C, wh, whc;

@@ -118,6 +118,12 @@ __device__ __forceinline__ unsigned lanes_below(unsigned long long m, int lane) 
     return (unsigned)__popcll(m & ((1ull << lane) - 1ull));
 }
 
+// floor(n / d) for 0 <= n < 4096, 1 <= d <= 64: the reciprocal's relative
+// error (about 1e-7) stays far below the 0.5/d margin of n + 0.5.
+__device__ __forceinline__ int small_div(int n, int d) {
+    return (int)(((float)n + 0.5f) * __builtin_amdgcn_rcpf((float)d));
+}
+
 // The `mid` smallest of list (ld, li)[0..m), m <= 64, in std::partial_sort
 // order, written to (od, oi)[0..mid).  Fast path: repeated wave arg-min; a
 // value equal to another remaining value at ranks < mid means the heap's
@@ -139,6 +145,41 @@ __device__ void wave_select(const float* ld, const uint32_t* li, int m, int mid,
             wave_sync();
             if (lane == 0) {
                 for (int q = 0; q < m; q++) { scr[q].d = ld[q]; scr[q].i = li[q]; }
+                vb_partial_sort(scr, m, mid);
+                for (int q = 0; q < mid; q++) { od[q] = scr[q].d; oi[q] = scr[q].i; }
+            }
+        }
+    }
+    wave_sync();
+}
+
+// The same selection over a sparse list: lane l holds entry l when `valid`
+// (bal = ballot of valid); the list is the valid entries in lane order.
+// Only the tie path compacts them (into cd, ci) for the exact replay.
+__device__ void wave_select_sparse(const float* ld, const uint32_t* li, bool valid, unsigned long long bal,
+                                   int mid, float* od, uint32_t* oi, float* cd, uint32_t* ci, VbEnt* scr,
+                                   int lane) {
+    if (mid > 0) {
+        unsigned key = valid ? fkey(ld[lane]) : 0xFFFFFFFFu;
+        bool tie = false;
+        for (int j = 0; j < mid; j++) {
+            const unsigned mk = wave_min_u32(key);
+            const unsigned long long b = __ballot(key == mk);
+            if (__popcll(b) != 1) { tie = true; break; }
+            const int pos = __ffsll((long long)b) - 1;
+            if (lane == 0) { od[j] = ld[pos]; oi[j] = li[pos]; }
+            if (lane == pos) key = 0xFFFFFFFFu;
+        }
+        if (tie) {
+            if (valid) {
+                const unsigned p = lanes_below(bal, lane);
+                cd[p] = ld[lane];
+                ci[p] = li[lane];
+            }
+            wave_sync();
+            const int m = __popcll(bal);
+            if (lane == 0) {
+                for (int q = 0; q < m; q++) { scr[q].d = cd[q]; scr[q].i = ci[q]; }
                 vb_partial_sort(scr, m, mid);
                 for (int q = 0; q < mid; q++) { od[q] = scr[q].d; oi[q] = scr[q].i; }
             }
@@ -190,7 +231,8 @@ __device__ __forceinline__ void bm_one(const A& a, const long r, float* base, co
     }
     ridx = __builtin_amdgcn_readfirstlane(ridx);
     slot = __builtin_amdgcn_readfirstlane(slot);
-    const unsigned rt = ridx / whc, ry = (ridx % wh) / W, rx = ridx % W;
+    const Div32 dW = {a.dW.m, a.dW.s1, a.dW.s2}, dWHC = {a.dWHC.m, a.dWHC.s1, a.dWHC.s2};
+    const unsigned rt = div32(ridx, dWHC), ry = div32(ridx - rt * whc, dW), rx = ridx - rt * whc - ry * W;
     const float norm = (float)(unsigned)(K * K * a.kt * C);
     constexpr int K2 = K * K;
 
@@ -291,6 +333,7 @@ __device__ __forceinline__ void bm_one(const A& a, const long r, float* base, co
     //    stored where the reference's sequential order puts them (all
     //    forward frames, then all backward frames).
     const int P2 = a.Npr * a.Npr, hs = (a.Npr - 1) / 2;
+    const int P2p = pow2ceil(P2), lg2 = __builtin_ctz(P2p);
     const int cs1 = L.tp1 * L.th1;
     const int nfw = (int)(min(rt + (unsigned)a.Nf, a.F - (unsigned)a.kt) - rt);
     const int nbw = (int)rt - max((int)rt - a.Nf, 0);
@@ -306,8 +349,8 @@ __device__ __forceinline__ void bm_one(const A& a, const long r, float* base, co
                 const int dir = ch == 0 ? 1 : -1;
                 const float* fl = ch == 0 ? a.ff : a.bf;
                 const unsigned mi = pp[(ch == 0 ? 8 : 0) + m];
-                unsigned px = mi % W, py = (mi % wh) / W;
-                const unsigned pt = mi / whc;
+                const unsigned pt = div32(mi, dWHC);
+                unsigned py = div32(mi - pt * whc, dW), px = mi - pt * whc - py * W;
                 const unsigned ft = ch == 0 ? pt : pt - 1;
                 const unsigned hk = K / 2;
                 const float fx = fl ? fl[ft * 2 * wh + (py + hk) * W + px + hk] : 0.f;
@@ -328,9 +371,9 @@ __device__ __forceinline__ void bm_one(const A& a, const long r, float* base, co
                 load_tile(tile + wn * C * cs1, L.tp1, cs1, win[wn * 5], win[wn * 5 + 1], win[wn * 5 + 2],
                           win[wn * 5 + 3], (unsigned)win[wn * 5 + 4]);
         wave_sync();
-        // slots: chain ch, window m, candidate l -> (ch * Nb + m) * P2 + l;
+        // slots: chain ch, window m, candidate l -> (ch * Nb + m) * P2p + l;
         // lane evaluates slots b0 + lane and b0 + 64 + lane together
-        const int nslots = 2 * Nb * P2;
+        const int nslots = 2 * Nb * P2p;
         for (int b0 = 0; b0 < nslots; b0 += 128) {
             bool valid[2] = {false, false};
             int toff[2] = {0, 0}, qxs[2] = {0, 0}, qys[2] = {0, 0};
@@ -339,11 +382,11 @@ __device__ __forceinline__ void bm_one(const A& a, const long r, float* base, co
             for (int h = 0; h < 2; h++) {
                 const int s = b0 + 64 * h + lane;
                 if (s < nslots) {
-                    const int wn = s / P2, l = s - wn * P2, ch = wn / Nb, m = wn - ch * Nb;
+                    const int wn = s >> lg2, l = s & (P2p - 1), ch = wn >= Nb ? 1 : 0, m = wn - ch * Nb;
                     if (act[ch]) {
                         const int x0 = win[wn * 5], y0 = win[wn * 5 + 1], wx = win[wn * 5 + 2], wy = win[wn * 5 + 3];
                         if (l < wx * wy) {
-                            const int lx = l % wx, ly = l / wx;
+                            const int ly = small_div(l, wx), lx = l - ly * wx;
                             const int qx = x0 + lx, qy = y0 + ly;
                             // already seen = inside an earlier window of this frame
                             bool seen = false;
@@ -385,19 +428,15 @@ __device__ __forceinline__ void bm_one(const A& a, const long r, float* base, co
             // top Nb of each window's list, appended in window order
             int nfb = 0;
             for (int m = 0; m < Nb; m++) {
-                const int sb = (ch * Nb + m) * P2;
-                const bool v = lane < P2 && s_v[sb + lane];
+                const int sb = (ch * Nb + m) * P2p;
+                const bool v = lane < P2p && s_v[sb + lane];
                 const unsigned long long bal = __ballot(v);
-                if (v) {
-                    const unsigned pos = lanes_below(bal, lane);
-                    c_d[pos] = s_d[sb + lane];
-                    c_i[pos] = s_i[sb + lane];
-                }
                 const int mm = __popcll(bal);
                 n_eval += mm;
-                wave_sync();
                 const int nb = min(Nb, mm);
-                wave_select(c_d, c_i, mm, nb, o_d, o_i, scr, lane);
+                // candidates in window raster order; the list is only
+                // compacted when the exact heap replay is needed
+                wave_select_sparse(s_d + sb, s_i + sb, v, bal, nb, o_d, o_i, c_d, c_i, scr, lane);
                 if (lane < nb) {
                     f_d[nfb + lane] = o_d[lane];
                     f_i[nfb + lane] = o_i[lane];
@@ -659,6 +698,7 @@ __device__ __forceinline__ void filter_one(const A& a, const long r, float* X, c
     const bool bior = a.T2D == 1;
     const bool had = a.T3D == 2;
     const uint32_t* mem = a.idx + (size_t)slot * a.N;
+    const Div32 dW = {a.dW.m, a.dW.s1, a.dW.s2}, dWHC = {a.dWHC.m, a.dWHC.s1, a.dWHC.s2};
     typedef const __attribute__((address_space(4))) float cfloat;
     const cfloat* TF = (const cfloat*)(bior ? a.tabs->biorF : a.tabs->dctF);
     const cfloat* TI = (const cfloat*)(bior ? a.tabs->biorI : a.tabs->dctI);
@@ -680,7 +720,8 @@ __device__ __forceinline__ void filter_one(const A& a, const long r, float* X, c
                 const bool act = lact && n < nsx;
                 if (act) {
                     const unsigned mi = mem[n];
-                    const unsigned px = mi % W, py = (mi % wh) / W, pt = mi / whc;
+                    const unsigned pt = div32(mi, dWHC);
+                    const unsigned py = div32(mi - pt * whc, dW), px = mi - pt * whc - py * W;
                     const float* row = src + pt * whc + c * wh + (py + rr) * W + px;
                     float x[K];
 #pragma unroll
