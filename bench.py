@@ -18,7 +18,8 @@ profiles/traffic.json when one matches the workload.  `hbm` is the whole
 job's algorithmic HBM rate (20*C bytes per output pixel) against the HBM
 peak.  `cpu_baseline` is the CPU oracle (an op-order-faithful C++/OpenMP
 restatement of the reference path) timed on this host on a sample of the
-same workload.
+same workload; beside it, `fp64_error` is the GPU path's per-step error
+against the oracle's FP64 replay on a small clip.
 """
 from __future__ import annotations
 
@@ -60,6 +61,30 @@ def cpu_baseline(noisy, sigma, frames, color=True):
             "sample": f"first {F} of the frames, {W}x{H}x{C}, sigma {sigma:g}, both steps, "
                       f"OpenMP over nb={nb} reference tiles (the reference's thread rule), "
                       f"{dt:.1f} s"}
+
+
+def fp64_error(api, synth):
+    """Per-step error of the GPU path against the oracle's FP64 replay of
+    the same step (same groups and threshold decisions, every operation
+    in double) on the 176x144x10 grey sigma-20 clip (seed 0): max |GPU -
+    FP64| of the basic estimate (step 1) and of the final estimate (step
+    2, both fed the GPU's basic estimate), on the 0-255 scale."""
+    from oracle import oracle as O
+    _, noisy = synth.noisy_clip(176, 144, 10, 1, 20.0, 0)
+    p1, p2 = api.initialize_parameters_1(20.0), api.initialize_parameters_2(20.0)
+
+    def ratio(n, d, fb):
+        return np.where(d == 0, fb, n / np.where(d == 0, 1, d))
+
+    n1, d1 = api.step(1, 20.0, noisy, None, p1)
+    basic = ratio(n1, d1, noisy).astype(np.float32)
+    m1, e1 = O.step_fp64(1, 20.0, noisy, None, p1)
+    n2, d2 = api.step(2, 20.0, noisy, basic, p2)
+    final = ratio(n2, d2, basic).astype(np.float32)
+    m2, e2 = O.step_fp64(2, 20.0, noisy, basic, p2)
+    return {"clip": "176x144x10 grey, sigma 20, seed 0",
+            "step1_max_abs": float(np.abs(basic - ratio(m1, e1, noisy)).max()),
+            "step2_max_abs": float(np.abs(final - ratio(m2, e2, basic)).max())}
 
 
 def main():
@@ -110,6 +135,7 @@ def main():
         print(f"[bench] cpu baseline on {args.cpu_frames} frames", file=sys.stderr, flush=True)
         _, noisy = synth.noisy_clip(W, H, min(args.cpu_frames, F), C, sigma, args.seed, vmax=args.vmax)
         res["cpu_baseline"] = cpu_baseline(noisy, sigma, min(args.cpu_frames, F))
+        res["fp64_error"] = fp64_error(api, synth)
     print(json.dumps(res), flush=True)
 
 

@@ -192,3 +192,34 @@ def test_run_chunked_ring(cfg1, oracle, gpu_ctx, monkeypatch):
     monkeypatch.setenv("VBM3D_EST_BUDGET_MB", "1")
     with V.Context(0) as ctx:
         _run_both(noisy, 20.0, oracle, ctx)
+
+
+def test_run_config2_threaded_grid(oracle, gpu_ctx):
+    """30 RGB frames 960x540, sigma 20, seed 1, against the oracle run with
+    the OpenMP grid of 16 threads (ref_grid_nb = 16), bit for bit."""
+    import vbm3d_amd as V
+    clean, noisy = synth.noisy_clip(960, 540, 30, 3, 20.0, 1)
+    p1, p2 = V.initialize_parameters_1(20.0), V.initialize_parameters_2(20.0)
+    gb, gf = V.denoise(noisy, 20.0, True, 16, ctx=gpu_ctx)
+    ob, of, _ = oracle.run_vbm3d(20.0, noisy, p1, p2, True, 16, 16)
+    assert_bits_equal(gb, ob, "basic")
+    assert_bits_equal(gf, of, "final")
+    assert synth.psnr(clean, gf) > synth.psnr(clean, noisy) + 10
+
+
+def test_step_error_against_fp64(cfg1, oracle, gpu_ctx):
+    """Per-step error of the GPU path against the FP64 replay of the same
+    step: within 1e-3 on the 0-255 scale."""
+    import vbm3d_amd as V
+    _, noisy = cfg1
+    p1, p2 = V.initialize_parameters_1(20.0), V.initialize_parameters_2(20.0)
+    n1, d1 = V.step(1, 20.0, noisy, None, p1, ctx=gpu_ctx)
+    basic = np.where(d1 == 0, noisy, n1 / np.where(d1 == 0, 1, d1)).astype(np.float32)
+    m1, e1 = oracle.step_fp64(1, 20.0, noisy, None, p1)
+    err1 = np.abs(basic - np.where(e1 == 0, noisy, m1 / np.where(e1 == 0, 1, e1))).max()
+    n2, d2 = V.step(2, 20.0, noisy, basic, p2, ctx=gpu_ctx)
+    final = np.where(d2 == 0, basic, n2 / np.where(d2 == 0, 1, d2))
+    m2, e2 = oracle.step_fp64(2, 20.0, noisy, basic, p2)
+    err2 = np.abs(final - np.where(e2 == 0, basic, m2 / np.where(e2 == 0, 1, e2))).max()
+    print(f"max |GPU - FP64|: step 1 {err1:.3g}, step 2 {err2:.3g}")
+    assert err1 < 1e-3 and err2 < 1e-3
