@@ -283,6 +283,21 @@ __device__ __forceinline__ void bm_one(const A& a, const long r, float* base, co
         ra = acc;
         rb = acd;
     };
+    // one candidate, register-lean (rolled over channels): reference from LDS
+    auto ssd1 = [&](const float* t, int tp, int cs) -> float {
+        float acc = 0.f;
+#pragma unroll 1
+        for (int c = 0; c < C; ++c)
+#pragma unroll 2
+            for (int y = 0; y < K; ++y) {
+#pragma unroll
+                for (int x = 0; x < K; ++x) {
+                    const float dif = refp[(c * K + y) * K + x] - t[c * cs + y * tp + x];
+                    acc = acc + dif * dif;
+                }
+            }
+        return acc;
+    };
     // copy the (wx+K-1) x (wy+K-1) area at (x0, y0) of frame tf into a
     // tile; lanes cover 4 rows of 16 columns (tile rows are at most 15 wide)
     const int lrow = lane >> 4, lcol = lane & 15;
@@ -339,7 +354,143 @@ __device__ __forceinline__ void bm_one(const A& a, const long r, float* base, co
     const int nbw = (int)rt - max((int)rt - a.Nf, 0);
     const int nst = max(nfw, nbw);
     const int Nb = a.Nb;
-    for (int f = 0; f < nst; f++) {
+    // Pipelined form (at most 2 matches per frame, windows of at most 32
+    // slots and 12 tile rows): the chains take turns, and each chain's
+    // tile loads for its next frame are issued into registers right after
+    // its selection, so they fly while the other chain computes.
+    const bool piped = Nb <= 2 && Nb * P2p <= 64 && L.th1 <= 12;
+    if (piped) {
+        float preF[2][C][3], preB[2][C][3];  // [window][channel][row group] per chain
+        int nfr[2] = {nfw, nbw};
+        // window of match m of chain ch for the next frame (flow projection
+        // of the current match, x first, then y at the updated x), lanes < Nb
+        auto windows = [&](int ch) {
+            if (lane < Nb) {
+                const int wn = ch * Nb + lane;
+                const int dir = ch == 0 ? 1 : -1;
+                const float* fl = ch == 0 ? a.ff : a.bf;
+                const unsigned mi = pp[(ch == 0 ? 8 : 0) + lane];
+                const unsigned pt = div32(mi, dWHC);
+                unsigned py = div32(mi - pt * whc, dW), px = mi - pt * whc - py * W;
+                const unsigned ft = ch == 0 ? pt : pt - 1;
+                const unsigned hk = K / 2;
+                const float fx = fl ? fl[ft * 2 * wh + (py + hk) * W + px + hk] : 0.f;
+                px = (unsigned)min(max((int)((float)px + roundf(fx)), 0), (int)(W - K));
+                const float fy = fl ? fl[ft * 2 * wh + wh + (py + hk) * W + px + hk] : 0.f;
+                py = (unsigned)min(max((int)((float)py + roundf(fy)), 0), (int)(H - K));
+                const int x0 = max(0, (int)px - hs), y0 = max(0, (int)py - hs);
+                win[wn * 5 + 0] = x0;
+                win[wn * 5 + 1] = y0;
+                win[wn * 5 + 2] = min((int)W - K, (int)px + hs) - x0 + 1;
+                win[wn * 5 + 3] = min((int)H - K, (int)py + hs) - y0 + 1;
+                win[wn * 5 + 4] = (int)(pt + dir);
+            }
+            wave_sync();
+        };
+        // the areas of chain ch's windows: lanes cover 4 rows x 16 columns
+        auto issue = [&](int ch, float (&pre)[2][C][3]) {
+#pragma unroll
+            for (int m = 0; m < 2; m++) {
+                if (m >= Nb) break;
+                const int* w5 = win + (ch * Nb + m) * 5;
+                const int aw = w5[2] + K - 1, ah = w5[3] + K - 1;
+                const unsigned fb = (unsigned)w5[4] * whc + (unsigned)w5[1] * W + (unsigned)w5[0];
+#pragma unroll
+                for (int c = 0; c < C; c++)
+#pragma unroll
+                    for (int g = 0; g < 3; g++) {
+                        const int ty = lrow + 4 * g;
+                        pre[m][c][g] = (lcol < aw && ty < ah) ? vid[fb + c * wh + ty * W + lcol] : 0.f;
+                    }
+            }
+        };
+        auto store = [&](int ch, const float (&pre)[2][C][3]) {
+#pragma unroll
+            for (int m = 0; m < 2; m++) {
+                if (m >= Nb) break;
+                const int wn = ch * Nb + m;
+                const int aw = win[wn * 5 + 2] + K - 1, ah = win[wn * 5 + 3] + K - 1;
+                float* t = tile + wn * C * cs1;
+#pragma unroll
+                for (int c = 0; c < C; c++)
+#pragma unroll
+                    for (int g = 0; g < 3; g++) {
+                        const int ty = lrow + 4 * g;
+                        if (lcol < aw && ty < ah) t[c * cs1 + ty * L.tp1 + lcol] = pre[m][c][g];
+                    }
+            }
+            wave_sync();
+        };
+        for (int ch = 0; ch < 2; ch++)
+            if (nfr[ch] > 0) {
+                windows(ch);
+                if (ch == 0) issue(0, preF);
+                else issue(1, preB);
+            }
+        for (int f = 0; f < nst; f++) {
+            for (int ch = 0; ch < 2; ch++) {
+                if (f >= nfr[ch]) continue;
+                if (ch == 0) store(0, preF);
+                else store(1, preB);
+                // one candidate per lane: slot = m * P2p + l
+                {
+                    const int s = lane, m = s >> lg2, l = s & (P2p - 1);
+                    bool valid = false;
+                    if (m < Nb) {
+                        const int wn = ch * Nb + m;
+                        const int x0 = win[wn * 5], y0 = win[wn * 5 + 1], wx = win[wn * 5 + 2], wy = win[wn * 5 + 3];
+                        if (l < wx * wy) {
+                            const int ly = small_div(l, wx), lx = l - ly * wx;
+                            const int qx = x0 + lx, qy = y0 + ly;
+                            bool seen = false;
+                            for (int m2 = 0; m2 < m; m2++) {
+                                const int* w2 = win + (ch * Nb + m2) * 5;
+                                seen |= qx >= w2[0] && qx < w2[0] + w2[2] && qy >= w2[1] && qy < w2[1] + w2[3];
+                            }
+                            valid = !seen;
+                            if (valid) {
+                                const float sv = ssd1(tile + wn * C * cs1 + ly * L.tp1 + lx, L.tp1, cs1);
+                                s_d[s] = sv / norm - (((unsigned)qx == rx && (unsigned)qy == ry) ? a.d : 0.f);
+                                s_i[s] = (unsigned)win[wn * 5 + 4] * whc + qy * W + qx;
+                            }
+                        }
+                    }
+                    s_v[s] = valid;
+                }
+                wave_sync();
+                int nfb = 0;
+                for (int m = 0; m < Nb; m++) {
+                    const int sb = m * P2p;
+                    const bool v = lane < P2p && s_v[sb + lane];
+                    const unsigned long long bal = __ballot(v);
+                    const int mm = __popcll(bal);
+                    n_eval += mm;
+                    const int nb = min(Nb, mm);
+                    wave_select_sparse(s_d + sb, s_i + sb, v, bal, nb, o_d, o_i, c_d, c_i, scr, lane);
+                    if (lane < nb) {
+                        f_d[nfb + lane] = o_d[lane];
+                        f_i[nfb + lane] = o_i[lane];
+                    }
+                    nfb += nb;
+                    wave_sync();
+                }
+                wave_select(f_d, f_i, nfb, Nb, o_d, o_i, scr, lane);
+                const int at = Nb * (1 + (ch == 0 ? f : nfw + f));
+                if (lane < Nb) {
+                    pp[(ch == 0 ? 8 : 0) + lane] = o_i[lane];
+                    b_d[at + lane] = o_d[lane];
+                    b_i[at + lane] = o_i[lane];
+                }
+                wave_sync();
+                if (f + 1 < nfr[ch]) {
+                    windows(ch);
+                    if (ch == 0) issue(0, preF);
+                    else issue(1, preB);
+                }
+            }
+        }
+    }
+    for (int f = 0; f < (piped ? 0 : nst); f++) {
         const bool act[2] = {f < nfw, f < nbw};  // [0] forward, [1] backward
         // windows: flow projection of each temp match (x first, then y at
         // the updated x), clamped Npr x Npr box in the next frame
@@ -481,8 +632,15 @@ __device__ __forceinline__ void bm_one(const A& a, const long r, float* base, co
     }
 }
 
+// Occupancy floor per instantiation (waves per SIMD the register
+// allocator must leave room for), measured: at k = 7 with three channels
+// the kernel runs faster held to 4 waves (a few spills), elsewhere faster
+// unconstrained.
 template <int K, int C>
-__global__ __launch_bounds__(256) void k_bm(BMArgs a) {
+constexpr int bm_waves_floor() { return (K == 7 && C == 3) ? 4 : 1; }
+
+template <int K, int C>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(bm_waves_floor<K, C>()))) void k_bm(BMArgs a) {
     extern __shared__ float smem[];
     const BMLay L = bm_layout(K, C, a.Ns, a.Npr, a.Nb);
     const int lane = threadIdx.x & 63;
@@ -494,7 +652,7 @@ __global__ __launch_bounds__(256) void k_bm(BMArgs a) {
 }
 
 template <int K, int C>
-__global__ __launch_bounds__(256) void k_bm_multi(const BMArgs* __restrict__ jobs, WorkQueue q, int words) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(bm_waves_floor<K, C>()))) void k_bm_multi(const BMArgs* __restrict__ jobs, WorkQueue q, int words) {
     extern __shared__ float smem[];
     const int lane = threadIdx.x & 63;
     const long g = uniform_long((long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
