@@ -866,6 +866,25 @@ __device__ __forceinline__ void filter_one(const A& a, const long r, float* X, c
     const int nl = lane / K, rr = lane - nl * K;    // batch patch, row / column index
     const bool lact = lane < PB * K;
     float wsel = 0.f;  // step 1: lane c ends with channel c's weight
+    // row rr of member nl of the first lane batch: its base offset, and the
+    // row of the next (channel, source) pair fetched ahead of its use
+    const bool act0 = lact && nl < nsx;
+    unsigned mb = 0;
+    if (act0) {
+        const unsigned mi = mem[nl];
+        const unsigned pt = div32(mi, dWHC);
+        const unsigned py = div32(mi - pt * whc, dW), px = mi - pt * whc - py * W;
+        mb = pt * whc + (py + rr) * W + px;
+    }
+    float xn[K];
+    auto fetch = [&](int c, int g) {
+        if (act0) {
+            const float* row = (g == 0 ? a.noisy : a.basic) + mb + c * wh;
+#pragma unroll
+            for (int q = 0; q < K; q++) xn[q] = row[q];
+        }
+    };
+    fetch(0, 0);
 
     for (int c = 0; c < C; c++) {
         // ---- forward 2D transforms of all members (bior_2d_forward
@@ -877,13 +896,22 @@ __device__ __forceinline__ void filter_one(const A& a, const long r, float* X, c
                 const int n = nb0 + nl;
                 const bool act = lact && n < nsx;
                 if (act) {
-                    const unsigned mi = mem[n];
-                    const unsigned pt = div32(mi, dWHC);
-                    const unsigned py = div32(mi - pt * whc, dW), px = mi - pt * whc - py * W;
-                    const float* row = src + pt * whc + c * wh + (py + rr) * W + px;
                     float x[K];
+                    if (nb0 == 0) {
+                        // the first batch's row was fetched ahead; fetch the
+                        // next (channel, source) row while this one is used
 #pragma unroll
-                    for (int q = 0; q < K; q++) x[q] = row[q];
+                        for (int q = 0; q < K; q++) x[q] = xn[q];
+                        if (STEP == 2 && g == 0) fetch(c, 1);
+                        else if (c + 1 < C) fetch(c + 1, 0);
+                    } else {
+                        const unsigned mi = mem[n];
+                        const unsigned pt = div32(mi, dWHC);
+                        const unsigned py = div32(mi - pt * whc, dW), px = mi - pt * whc - py * W;
+                        const float* row = src + pt * whc + c * wh + (py + rr) * W + px;
+#pragma unroll
+                        for (int q = 0; q < K; q++) x[q] = row[q];
+                    }
                     // pass 1 along the row: t[v] = sum_q x[q] M[v][q]
 #pragma unroll
                     for (int v = 0; v < K; v++) {
