@@ -55,6 +55,7 @@ struct Tabs {
 // pitch so that lanes reading neighbouring candidates hit different banks.
 struct BMLay {
     int th0, tp0, th1, tp1;
+    bool piped;  // the predictive search runs chain by chain in one tile region
     int ref, tile, sd, si, sv, cd, ci, bd, bi, fd, fi, od, oi, scr, pp, win, total;
 };
 
@@ -66,9 +67,13 @@ __host__ __device__ inline BMLay bm_layout(int K, int C, int Ns, int Npr, int Nb
     L.th1 = Npr + K - 1;
     L.tp1 = L.th1 | 1;
     // predictive-stage slots: window w owns [w * P2p, (w + 1) * P2p),
-    // P2p = Npr^2 rounded up to a power of two
-    const int S = ((2 * Nb * pow2ceil(Npr * Npr) + 127) / 128) * 128;
-    const int t0 = L.th0 * L.tp0 * C, t1 = 2 * Nb * L.th1 * L.tp1 * C;
+    // P2p = Npr^2 rounded up to a power of two.  The pipelined form (at
+    // most 2 matches per frame, one slot per lane, at most 12 tile rows)
+    // holds one chain's windows at a time.
+    const int P2p = pow2ceil(Npr * Npr);
+    L.piped = Nb <= 2 && Nb * P2p <= 64 && L.th1 <= 12;
+    const int S = L.piped ? 64 : ((2 * Nb * P2p + 127) / 128) * 128;
+    const int t0 = L.th0 * L.tp0 * C, t1 = (L.piped ? 1 : 2) * Nb * L.th1 * L.tp1 * C;
     int t = t0 > t1 ? t0 : t1;
     t = t > 128 ? t : 128;  // the slow-path scratch (64 entries) reuses the tile
     L.ref = o; o += K * K * C;

@@ -355,13 +355,26 @@ __device__ __forceinline__ void bm_one(const A& a, const long r, float* base, co
     const int nst = max(nfw, nbw);
     const int Nb = a.Nb;
     // Pipelined form (at most 2 matches per frame, windows of at most 32
-    // slots and 12 tile rows): the chains take turns, and each chain's
-    // tile loads for its next frame are issued into registers right after
-    // its selection, so they fly while the other chain computes.
-    const bool piped = Nb <= 2 && Nb * P2p <= 64 && L.th1 <= 12;
+    // slots and 12 tile rows).  Tasks (chain, frame) run in the order
+    // forward 0, backward 0, forward 1, ...; when consecutive tasks belong
+    // to different chains, the tile loads of the task after next (same
+    // chain as the current one, whose windows its selection has just
+    // fixed) are issued into registers and fly while the next task
+    // computes.  One tile region serves all tasks.
+    const bool piped = L.piped;
     if (piped) {
-        float preF[2][C][3], preB[2][C][3];  // [window][channel][row group] per chain
-        int nfr[2] = {nfw, nbw};
+        float pre[2][C][3];  // [window][channel][row group] of the task in flight
+        const int nfr[2] = {nfw, nbw};
+        const int ntask = nfw + nbw;
+        auto task = [&](int i, int& ch, int& f) {
+            int n = 0;
+            for (int ff = 0; ff < nst; ff++)
+                for (int cc = 0; cc < 2; cc++)
+                    if (ff < nfr[cc]) {
+                        if (n == i) { ch = cc; f = ff; return; }
+                        n++;
+                    }
+        };
         // window of match m of chain ch for the next frame (flow projection
         // of the current match, x first, then y at the updated x), lanes < Nb
         auto windows = [&](int ch) {
@@ -388,7 +401,7 @@ __device__ __forceinline__ void bm_one(const A& a, const long r, float* base, co
             wave_sync();
         };
         // the areas of chain ch's windows: lanes cover 4 rows x 16 columns
-        auto issue = [&](int ch, float (&pre)[2][C][3]) {
+        auto issue = [&](int ch) {
 #pragma unroll
             for (int m = 0; m < 2; m++) {
                 if (m >= Nb) break;
@@ -404,13 +417,13 @@ __device__ __forceinline__ void bm_one(const A& a, const long r, float* base, co
                     }
             }
         };
-        auto store = [&](int ch, const float (&pre)[2][C][3]) {
+        auto store = [&](int ch) {
 #pragma unroll
             for (int m = 0; m < 2; m++) {
                 if (m >= Nb) break;
                 const int wn = ch * Nb + m;
                 const int aw = win[wn * 5 + 2] + K - 1, ah = win[wn * 5 + 3] + K - 1;
-                float* t = tile + wn * C * cs1;
+                float* t = tile + m * C * cs1;
 #pragma unroll
                 for (int c = 0; c < C; c++)
 #pragma unroll
@@ -421,71 +434,89 @@ __device__ __forceinline__ void bm_one(const A& a, const long r, float* base, co
             }
             wave_sync();
         };
-        for (int ch = 0; ch < 2; ch++)
-            if (nfr[ch] > 0) {
-                windows(ch);
-                if (ch == 0) issue(0, preF);
-                else issue(1, preB);
+        if (nfw > 0) windows(0);
+        if (nbw > 0) windows(1);
+        int pend = -1;  // task whose tiles are in `pre`
+        if (ntask > 0) {
+            int c0, f0;
+            task(0, c0, f0);
+            issue(c0);
+            store(c0);
+            if (ntask > 1) {
+                int c1, f1;
+                task(1, c1, f1);
+                if (c1 != c0) { issue(c1); pend = 1; }
             }
-        for (int f = 0; f < nst; f++) {
-            for (int ch = 0; ch < 2; ch++) {
-                if (f >= nfr[ch]) continue;
-                if (ch == 0) store(0, preF);
-                else store(1, preB);
-                // one candidate per lane: slot = m * P2p + l
-                {
-                    const int s = lane, m = s >> lg2, l = s & (P2p - 1);
-                    bool valid = false;
-                    if (m < Nb) {
-                        const int wn = ch * Nb + m;
-                        const int x0 = win[wn * 5], y0 = win[wn * 5 + 1], wx = win[wn * 5 + 2], wy = win[wn * 5 + 3];
-                        if (l < wx * wy) {
-                            const int ly = small_div(l, wx), lx = l - ly * wx;
-                            const int qx = x0 + lx, qy = y0 + ly;
-                            bool seen = false;
-                            for (int m2 = 0; m2 < m; m2++) {
-                                const int* w2 = win + (ch * Nb + m2) * 5;
-                                seen |= qx >= w2[0] && qx < w2[0] + w2[2] && qy >= w2[1] && qy < w2[1] + w2[3];
-                            }
-                            valid = !seen;
-                            if (valid) {
-                                const float sv = ssd1(tile + wn * C * cs1 + ly * L.tp1 + lx, L.tp1, cs1);
-                                s_d[s] = sv / norm - (((unsigned)qx == rx && (unsigned)qy == ry) ? a.d : 0.f);
-                                s_i[s] = (unsigned)win[wn * 5 + 4] * whc + qy * W + qx;
-                            }
+        }
+        for (int ti = 0; ti < ntask; ti++) {
+            int ch, f;
+            task(ti, ch, f);
+            // one candidate per lane: slot = m * P2p + l
+            {
+                const int s = lane, m = s >> lg2, l = s & (P2p - 1);
+                bool valid = false;
+                if (m < Nb) {
+                    const int wn = ch * Nb + m;
+                    const int x0 = win[wn * 5], y0 = win[wn * 5 + 1], wx = win[wn * 5 + 2], wy = win[wn * 5 + 3];
+                    if (l < wx * wy) {
+                        const int ly = small_div(l, wx), lx = l - ly * wx;
+                        const int qx = x0 + lx, qy = y0 + ly;
+                        bool seen = false;
+                        for (int m2 = 0; m2 < m; m2++) {
+                            const int* w2 = win + (ch * Nb + m2) * 5;
+                            seen |= qx >= w2[0] && qx < w2[0] + w2[2] && qy >= w2[1] && qy < w2[1] + w2[3];
+                        }
+                        valid = !seen;
+                        if (valid) {
+                            const float sv = ssd1(tile + m * C * cs1 + ly * L.tp1 + lx, L.tp1, cs1);
+                            s_d[s] = sv / norm - (((unsigned)qx == rx && (unsigned)qy == ry) ? a.d : 0.f);
+                            s_i[s] = (unsigned)win[wn * 5 + 4] * whc + qy * W + qx;
                         }
                     }
-                    s_v[s] = valid;
                 }
+                s_v[s] = valid;
+            }
+            wave_sync();
+            int nfb = 0;
+            for (int m = 0; m < Nb; m++) {
+                const int sb = m * P2p;
+                const bool v = lane < P2p && s_v[sb + lane];
+                const unsigned long long bal = __ballot(v);
+                const int mm = __popcll(bal);
+                n_eval += mm;
+                const int nb = min(Nb, mm);
+                wave_select_sparse(s_d + sb, s_i + sb, v, bal, nb, o_d, o_i, c_d, c_i, scr, lane);
+                if (lane < nb) {
+                    f_d[nfb + lane] = o_d[lane];
+                    f_i[nfb + lane] = o_i[lane];
+                }
+                nfb += nb;
                 wave_sync();
-                int nfb = 0;
-                for (int m = 0; m < Nb; m++) {
-                    const int sb = m * P2p;
-                    const bool v = lane < P2p && s_v[sb + lane];
-                    const unsigned long long bal = __ballot(v);
-                    const int mm = __popcll(bal);
-                    n_eval += mm;
-                    const int nb = min(Nb, mm);
-                    wave_select_sparse(s_d + sb, s_i + sb, v, bal, nb, o_d, o_i, c_d, c_i, scr, lane);
-                    if (lane < nb) {
-                        f_d[nfb + lane] = o_d[lane];
-                        f_i[nfb + lane] = o_i[lane];
-                    }
-                    nfb += nb;
-                    wave_sync();
+            }
+            wave_select(f_d, f_i, nfb, Nb, o_d, o_i, scr, lane);
+            const int at = Nb * (1 + (ch == 0 ? f : nfw + f));
+            if (lane < Nb) {
+                pp[(ch == 0 ? 8 : 0) + lane] = o_i[lane];
+                b_d[at + lane] = o_d[lane];
+                b_i[at + lane] = o_i[lane];
+            }
+            wave_sync();
+            if (f + 1 < nfr[ch]) windows(ch);
+            // the next task's tiles into the (now free) tile region
+            if (ti + 1 < ntask) {
+                int cn, fn;
+                task(ti + 1, cn, fn);
+                if (pend == ti + 1) {
+                    store(cn);
+                } else {
+                    issue(cn);
+                    store(cn);
                 }
-                wave_select(f_d, f_i, nfb, Nb, o_d, o_i, scr, lane);
-                const int at = Nb * (1 + (ch == 0 ? f : nfw + f));
-                if (lane < Nb) {
-                    pp[(ch == 0 ? 8 : 0) + lane] = o_i[lane];
-                    b_d[at + lane] = o_d[lane];
-                    b_i[at + lane] = o_i[lane];
-                }
-                wave_sync();
-                if (f + 1 < nfr[ch]) {
-                    windows(ch);
-                    if (ch == 0) issue(0, preF);
-                    else issue(1, preB);
+                pend = -1;
+                if (ti + 2 < ntask) {
+                    int c2, f2;
+                    task(ti + 2, c2, f2);
+                    if (c2 == ch && cn != ch) { issue(c2); pend = ti + 2; }
                 }
             }
         }
