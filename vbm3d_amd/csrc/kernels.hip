@@ -664,11 +664,11 @@ __device__ __forceinline__ void bm_one(const A& a, const long r, float* base, co
 }
 
 // Occupancy floor per instantiation (waves per SIMD the register
-// allocator must leave room for), measured: at k = 7 with three channels
-// the kernel runs faster held to 4 waves (a few spills), elsewhere faster
-// unconstrained.
+// allocator must leave room for), measured: with three channels the kernel
+// runs fastest held to 5 waves (96 registers, a few spills; 4 and 6 are
+// slower), with one channel unconstrained.
 template <int K, int C>
-constexpr int bm_waves_floor() { return (K == 7 && C == 3) ? 4 : 1; }
+constexpr int bm_waves_floor() { return C == 3 ? 5 : 1; }
 
 template <int K, int C>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(bm_waves_floor<K, C>()))) void k_bm(BMArgs a) {
