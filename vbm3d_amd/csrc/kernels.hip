@@ -790,12 +790,21 @@ __device__ __forceinline__ void t3_inv(float* v, bool had) {
     else haar_inv<NS>(v);
 }
 
+// Column of (member n, coefficient kk) in row kk of the Wiener v^2 buffer:
+// n rotated by kk / (64 / NS), so that the 64 lanes' stores of one n fall
+// in 64 distinct banks while each row stays contiguous.
+template <int NS>
+__host__ __device__ constexpr int v2_col(int kk, int n) {
+    return (n + (kk >> (NS == 1 ? 6 : NS == 2 ? 5 : NS == 4 ? 4 : NS == 8 ? 3 : 2))) & (NS - 1);
+}
+
 // 1D transform along the group and shrinkage for one channel
 // (ht_filtering_haar/_hadamard src/vbm3d.cpp:1405-1521,
 // wiener_filtering_haar/_hadamard :1539-1666).  Lane kk holds coefficient
 // kk of every group member: zg/eg are [n][64].  Step 1 returns the
-// aggregation weight (valid in every lane); step 2 stores value^2 at the
-// flat index n + kk*nSx of v2 for the weight sum done afterwards.
+// aggregation weight (valid in every lane); step 2 stores value^2 of
+// (n, kk) in row kk of v2, at column v2_col for the weight sum done
+// afterwards in the reference's flat order n + kk*nSx (wiener_sum).
 template <int K, int NS, int STEP>
 __device__ float shrink(float* zg, float* eg, float* v2, int lane, bool had, float lambda,
                         float sig) {
@@ -840,7 +849,7 @@ __device__ float shrink(float* zg, float* eg, float* v2, int lane, bool had, flo
             if (had) value = value * coef;
             value /= (value + s2);
             e[n] = had ? z[n] * value * coef : z[n] * value;
-            if (act) v2[lane * NS + n] = value * value;
+            if (act) v2[lane * NS + v2_col<NS>(lane, n)] = value * value;
         }
         t3_inv<NS>(e, had);
 #pragma unroll
@@ -850,17 +859,48 @@ __device__ float shrink(float* zg, float* eg, float* v2, int lane, bool had, flo
     return weight;
 }
 
+// Sequential float sum of channel c's v^2 over the flat index
+// f = n + kk*nSx (src/vbm3d.cpp:1651).  Rows are NS words, read 16 bytes
+// at a time; the rotation undone here is v2_col's.
+template <int K, int NS>
+__device__ __forceinline__ float wiener_sum(const float* vc) {
+    constexpr int K2 = K * K;
+    float acc = 0.f;
+#pragma unroll
+    for (int kk = 0; kk < K2; kk++) {
+        float x[NS];
+        if constexpr (NS >= 4) {
+#pragma unroll
+            for (int j = 0; j < NS / 4; j++) {
+                const float4 q = *(const float4*)(vc + kk * NS + 4 * j);
+                x[4 * j] = q.x;
+                x[4 * j + 1] = q.y;
+                x[4 * j + 2] = q.z;
+                x[4 * j + 3] = q.w;
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < NS; j++) x[j] = vc[kk * NS + j];
+        }
+#pragma unroll
+        for (int n = 0; n < NS; n++) acc = acc + x[v2_col<NS>(kk, n)];
+    }
+    return acc;
+}
+
 // Transposition buffer: patch n of a lane batch is a K x (K+1) tile at
 // n*xstride(K) (row pitch K+1).  The strides keep both the row-wise write and
 // the column-wise read of the 64 lanes (n, r) = n*K + r bank-conflict free
 // at K = 8 and within 2-way at K = 7.
 __host__ __device__ constexpr int xstride(int K) { return (K & 1) ? K * (K + 1) + 1 : K * (K + 1); }
 
-// Per-wavefront LDS layout of k_filter (4-byte words).
+// Per-wavefront LDS layout of k_filter (4-byte words): transposes, zg,
+// eg, then one v^2 buffer per channel, each region 16-byte aligned.
+__host__ __device__ constexpr int round4(int v) { return (v + 3) & ~3; }
 __host__ __device__ inline int filter_lds_words(int K, int C, int N, int step) {
     const int pb = 64 / K;
-    int w = pb * xstride(K) + N * K * K;
-    if (step == 2) w += N * K * K + C * K * K * N;
+    int w = round4(pb * xstride(K)) + round4(N * K * K);
+    if (step == 2) w += round4(N * K * K) + C * round4(K * K * N);
     return w;
 }
 
@@ -877,9 +917,10 @@ __device__ __forceinline__ void filter_one(const A& a, const long r, float* X, c
     constexpr int PB = 64 / K;           // patches per lane batch
     constexpr int XS = xstride(K), XP = K + 1;
     // X: [PB][XS] transposes, then the group buffers
-    float* zg = X + PB * XS;                       // [N][K2]
-    float* eg = zg + a.N * K2;                     // [N][K2] (step 2)
-    float* v2 = eg + a.N * K2;                     // [C][K2*N] (step 2)
+    const int gw = round4(a.N * K2);               // group buffer words
+    float* zg = X + round4(PB * XS);               // [N][K2]
+    float* eg = zg + gw;                           // [N][K2] (step 2)
+    float* v2 = eg + gw;                           // [C][gw] (step 2)
     const unsigned t = (unsigned)(r / a.nrf), i = (unsigned)(r - (long)t * a.nrf);
     const unsigned slot = __builtin_amdgcn_readfirstlane((t % a.ring) * a.nrf + i);
     const unsigned W = a.W, wh = a.wh, whc = a.whc;
@@ -973,7 +1014,7 @@ __device__ __forceinline__ void filter_one(const A& a, const long r, float* X, c
             }
         }
         // ---- 1D transform along the group, shrinkage, inverse
-        float* v2c = v2 + c * K2 * a.N;
+        float* v2c = v2 + c * gw;
         const float sgc = sig_of(a, c);
         float wc;
         switch (nsx) {
@@ -1028,9 +1069,15 @@ __device__ __forceinline__ void filter_one(const A& a, const long r, float* X, c
         // (src/vbm3d.cpp:1651), one lane per channel
         wave_sync();
         if (lane < C) {
-            const float* vc = v2 + lane * K2 * a.N;
-            float acc = 0.f;
-            for (int f = 0; f < K2 * nsx; f++) acc = acc + vc[f];
+            const float* vc = v2 + lane * gw;
+            float acc;
+            switch (nsx) {
+                case 1: acc = wiener_sum<K, 1>(vc); break;
+                case 2: acc = wiener_sum<K, 2>(vc); break;
+                case 4: acc = wiener_sum<K, 4>(vc); break;
+                case 8: acc = wiener_sum<K, 8>(vc); break;
+                default: acc = wiener_sum<K, 16>(vc); break;
+            }
             const float sg = sig_of(a, lane), s2 = sg * sg;
             float wgt;
             if (had) wgt = acc > 0.0f ? 1.0f / (s2 * acc) : 1.0f;
