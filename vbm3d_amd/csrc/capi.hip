@@ -788,7 +788,7 @@ int plan_batch(vbm3d_ctx* ctx, const vbm3d_clip* clips, int n, int nb, const vbm
                         cl->bm.push_back(bm);
                         cl->fa.push_back(fa);
                         cl->pre.push_back(cl->pre.back() + (bm.r_end - bm.r_begin));
-                        cl->wb = std::max(cl->wb, bm_lds_words((int)m.K, (int)m.C, p.Ns, p.Npr, p.Nb));
+                        cl->wb = std::max(cl->wb, bm_lds_words((int)m.K, (int)m.C, p.Ns, p.Npr, p.Nb, p.Nf));
                         cl->wf = std::max(cl->wf, filter_words((int)m.K, (int)m.C, p.N, st));
                     }
                     const unsigned e = std::min(m.Fr, a + sub.S);

@@ -55,11 +55,11 @@ struct Tabs {
 // pitch so that lanes reading neighbouring candidates hit different banks.
 struct BMLay {
     int th0, tp0, th1, tp1;
-    bool piped;  // the predictive search runs chain by chain in one tile region
+    bool piped;  // the paired form: both chains of a frame in shared passes
     int ref, tile, sd, si, sv, cd, ci, bd, bi, fd, fi, od, oi, scr, pp, win, total;
 };
 
-__host__ __device__ inline BMLay bm_layout(int K, int C, int Ns, int Npr, int Nb) {
+__host__ __device__ inline BMLay bm_layout(int K, int C, int Ns, int Npr, int Nb, int Nf) {
     BMLay L;
     int o = 0;
     L.th0 = Ns + K - 1;
@@ -67,31 +67,33 @@ __host__ __device__ inline BMLay bm_layout(int K, int C, int Ns, int Npr, int Nb
     L.th1 = Npr + K - 1;
     L.tp1 = L.th1 | 1;
     // predictive-stage slots: window w owns [w * P2p, (w + 1) * P2p),
-    // P2p = Npr^2 rounded up to a power of two.  The pipelined form (at
-    // most 2 matches per frame, one slot per lane, at most 12 tile rows)
-    // holds one chain's windows at a time.
+    // P2p = Npr^2 rounded up to a power of two.  The paired form (at most
+    // 2 matches per frame, one chain's slots within 64, at most 12 tile
+    // rows) gives chain h the slots [64 h, 64 h + 64), reads the reference
+    // patch through scalar loads and needs no validity array.
     const int P2p = pow2ceil(Npr * Npr);
     L.piped = Nb <= 2 && Nb * P2p <= 64 && L.th1 <= 12;
-    const int S = L.piped ? 64 : ((2 * Nb * P2p + 127) / 128) * 128;
-    const int t0 = L.th0 * L.tp0 * C, t1 = (L.piped ? 1 : 2) * Nb * L.th1 * L.tp1 * C;
+    const int S = L.piped ? 128 : ((2 * Nb * P2p + 127) / 128) * 128;
+    const int t0 = L.th0 * L.tp0 * C, t1 = 2 * Nb * L.th1 * L.tp1 * C;
     int t = t0 > t1 ? t0 : t1;
     t = t > 128 ? t : 128;  // the slow-path scratch (64 entries) reuses the tile
-    L.ref = o; o += K * K * C;
+    const int nbest = (Nb * (2 * Nf + 1) + 3) & ~3;  // picks of all frames
+    L.ref = o; o += L.piped ? 0 : K * K * C;
     L.tile = o; o += t;
     L.scr = L.tile;
     L.sd = o; o += S;
     L.si = o; o += S;
-    L.sv = o; o += S;
+    L.sv = o; o += L.piped ? 0 : S;
     L.cd = o; o += 64;
-    L.ci = o; o += 64;
-    L.bd = o; o += 64;
-    L.bi = o; o += 64;
-    L.fd = o; o += 64;
-    L.fi = o; o += 64;
+    L.ci = o; o += 64;  // directly after cd: the paired form's slot list spans both
+    L.bd = o; o += nbest;
+    L.bi = o; o += nbest;
+    L.fd = o; o += L.piped ? 8 : 64;
+    L.fi = o; o += L.piped ? 8 : 64;
     L.od = o; o += 16;
     L.oi = o; o += 16;
     L.pp = o; o += 16;
-    L.win = o; o += 80;
+    L.win = o; o += L.piped ? 20 : 80;
     L.total = o;
     return L;
 }
@@ -184,7 +186,7 @@ hipError_t launch_filter_multi(int K, int C, int step, const FArgs* jobs, const 
                                int waves, hipStream_t s);
 hipError_t launch_aggregate_queue(int C, const AArgs* jobs, const WorkQueue& q, int blocks, hipStream_t s);
 hipError_t launch_zero_u64(unsigned long long* p, int n, hipStream_t s);
-int bm_lds_words(int K, int C, int Ns, int Npr, int Nb);  // per wave, 4-byte words
+int bm_lds_words(int K, int C, int Ns, int Npr, int Nb, int Nf);  // per wave, 4-byte words
 int filter_words(int K, int C, int N, int step);
 hipError_t launch_adjoint_fallback(float* gz, const float* gy, const float* den, size_t n, hipStream_t s);
 hipError_t launch_sum_counts(const uint32_t* counts, long n, unsigned long long* stats, hipStream_t s);

@@ -200,7 +200,7 @@ __device__ void wave_select_sparse(const float* ld, const uint32_t* li, bool val
 // ---------------------------------------------------------------------
 template <int K, int C, class A>
 __device__ __forceinline__ void bm_one(const A& a, const long r, float* base, const int lane) {
-    const BMLay L = bm_layout(K, C, a.Ns, a.Npr, a.Nb);
+    const BMLay L = bm_layout(K, C, a.Ns, a.Npr, a.Nb, a.Nf);
     float* tile = base + L.tile;
     float* s_d = base + L.sd;
     uint32_t* s_i = (uint32_t*)(base + L.si);
@@ -240,8 +240,8 @@ __device__ __forceinline__ void bm_one(const A& a, const long r, float* base, co
     // address space so it arrives in scalar registers (SMEM loads) and
     // costs neither LDS bandwidth nor vector issue slots.
     typedef const __attribute__((address_space(4))) float cfloat;
-    float* refp = base + L.ref;  // the reference patch, [c][y][x]
-    for (int i = lane; i < K2 * C; i += 64) {
+    float* refp = base + L.ref;  // the reference patch, [c][y][x] (general form only)
+    for (int i = lane; i < (L.piped ? 0 : K2 * C); i += 64) {
         const int c = i / K2, e = i - c * K2;
         refp[i] = vid[ridx + c * wh + (e / K) * W + e % K];
     }
@@ -282,21 +282,6 @@ __device__ __forceinline__ void bm_one(const A& a, const long r, float* base, co
             }
         ra = acc;
         rb = acd;
-    };
-    // one candidate, register-lean (rolled over channels): reference from LDS
-    auto ssd1 = [&](const float* t, int tp, int cs) -> float {
-        float acc = 0.f;
-#pragma unroll 1
-        for (int c = 0; c < C; ++c)
-#pragma unroll 2
-            for (int y = 0; y < K; ++y) {
-#pragma unroll
-                for (int x = 0; x < K; ++x) {
-                    const float dif = refp[(c * K + y) * K + x] - t[c * cs + y * tp + x];
-                    acc = acc + dif * dif;
-                }
-            }
-        return acc;
     };
     // copy the (wx+K-1) x (wy+K-1) area at (x0, y0) of frame tf into a
     // tile; lanes cover 4 rows of 16 columns (tile rows are at most 15 wide)
@@ -354,171 +339,162 @@ __device__ __forceinline__ void bm_one(const A& a, const long r, float* base, co
     const int nbw = (int)rt - max((int)rt - a.Nf, 0);
     const int nst = max(nfw, nbw);
     const int Nb = a.Nb;
-    // Pipelined form (at most 2 matches per frame, windows of at most 32
-    // slots and 12 tile rows).  Tasks (chain, frame) run in the order
-    // forward 0, backward 0, forward 1, ...; when consecutive tasks belong
-    // to different chains, the tile loads of the task after next (same
-    // chain as the current one, whose windows its selection has just
-    // fixed) are issued into registers and fly while the next task
-    // computes.  One tile region serves all tasks.
+    // Paired form (at most 2 matches per frame, windows of at most 64
+    // slots per chain, at most 12 tile rows): frame f of the forward chain
+    // and frame f of the backward chain are searched together.  Chain h
+    // owns slots h*64 + m*P2p + l; the valid slots of both chains are
+    // listed in slot order and evaluated 64 per pass, so that the lanes
+    // are not left idle by windows smaller than the wave.
     const bool piped = L.piped;
     if (piped) {
-        float pre[2][C][3];  // [window][channel][row group] of the task in flight
         const int nfr[2] = {nfw, nbw};
-        const int ntask = nfw + nbw;
-        auto task = [&](int i, int& ch, int& f) {
-            int n = 0;
-            for (int ff = 0; ff < nst; ff++)
-                for (int cc = 0; cc < 2; cc++)
-                    if (ff < nfr[cc]) {
-                        if (n == i) { ch = cc; f = ff; return; }
-                        n++;
-                    }
-        };
-        // window of match m of chain ch for the next frame (flow projection
-        // of the current match, x first, then y at the updated x), lanes < Nb
-        auto windows = [&](int ch) {
-            if (lane < Nb) {
-                const int wn = ch * Nb + lane;
-                const int dir = ch == 0 ? 1 : -1;
-                const float* fl = ch == 0 ? a.ff : a.bf;
-                const unsigned mi = pp[(ch == 0 ? 8 : 0) + lane];
-                const unsigned pt = div32(mi, dWHC);
-                unsigned py = div32(mi - pt * whc, dW), px = mi - pt * whc - py * W;
-                const unsigned ft = ch == 0 ? pt : pt - 1;
-                const unsigned hk = K / 2;
-                const float fx = fl ? fl[ft * 2 * wh + (py + hk) * W + px + hk] : 0.f;
-                px = (unsigned)min(max((int)((float)px + roundf(fx)), 0), (int)(W - K));
-                const float fy = fl ? fl[ft * 2 * wh + wh + (py + hk) * W + px + hk] : 0.f;
-                py = (unsigned)min(max((int)((float)py + roundf(fy)), 0), (int)(H - K));
-                const int x0 = max(0, (int)px - hs), y0 = max(0, (int)py - hs);
-                win[wn * 5 + 0] = x0;
-                win[wn * 5 + 1] = y0;
-                win[wn * 5 + 2] = min((int)W - K, (int)px + hs) - x0 + 1;
-                win[wn * 5 + 3] = min((int)H - K, (int)py + hs) - y0 + 1;
-                win[wn * 5 + 4] = (int)(pt + dir);
+        uint32_t* lst = (uint32_t*)c_d;  // c_d and c_i, adjacent: 128 entries
+        // windows of frame f of both chains: flow projection of the chain's
+        // current matches (x first, then y at the updated x), clamped
+        // Npr x Npr box in the next frame; lanes < 2 Nb
+        auto windows = [&](int f) {
+            if (lane < 2 * Nb) {
+                const int ch = lane >= Nb ? 1 : 0, m = lane - ch * Nb;
+                if (f < nfr[ch]) {
+                    const int dir = ch == 0 ? 1 : -1;
+                    const float* fl = ch == 0 ? a.ff : a.bf;
+                    const unsigned mi = pp[(ch == 0 ? 8 : 0) + m];
+                    const unsigned pt = div32(mi, dWHC);
+                    unsigned py = div32(mi - pt * whc, dW), px = mi - pt * whc - py * W;
+                    const unsigned ft = ch == 0 ? pt : pt - 1;
+                    const unsigned hk = K / 2;
+                    const float fx = fl ? fl[ft * 2 * wh + (py + hk) * W + px + hk] : 0.f;
+                    px = (unsigned)min(max((int)((float)px + roundf(fx)), 0), (int)(W - K));
+                    const float fy = fl ? fl[ft * 2 * wh + wh + (py + hk) * W + px + hk] : 0.f;
+                    py = (unsigned)min(max((int)((float)py + roundf(fy)), 0), (int)(H - K));
+                    const int x0 = max(0, (int)px - hs), y0 = max(0, (int)py - hs);
+                    win[lane * 5 + 0] = x0;
+                    win[lane * 5 + 1] = y0;
+                    win[lane * 5 + 2] = min((int)W - K, (int)px + hs) - x0 + 1;
+                    win[lane * 5 + 3] = min((int)H - K, (int)py + hs) - y0 + 1;
+                    win[lane * 5 + 4] = (int)(pt + dir);
+                }
             }
             wave_sync();
         };
-        // the areas of chain ch's windows: lanes cover 4 rows x 16 columns
-        auto issue = [&](int ch) {
-#pragma unroll
-            for (int m = 0; m < 2; m++) {
-                if (m >= Nb) break;
-                const int* w5 = win + (ch * Nb + m) * 5;
-                const int aw = w5[2] + K - 1, ah = w5[3] + K - 1;
-                const unsigned fb = (unsigned)w5[4] * whc + (unsigned)w5[1] * W + (unsigned)w5[0];
-#pragma unroll
-                for (int c = 0; c < C; c++)
-#pragma unroll
-                    for (int g = 0; g < 3; g++) {
-                        const int ty = lrow + 4 * g;
-                        pre[m][c][g] = (lcol < aw && ty < ah) ? vid[fb + c * wh + ty * W + lcol] : 0.f;
-                    }
-            }
-        };
-        auto store = [&](int ch) {
-#pragma unroll
-            for (int m = 0; m < 2; m++) {
-                if (m >= Nb) break;
-                const int wn = ch * Nb + m;
-                const int aw = win[wn * 5 + 2] + K - 1, ah = win[wn * 5 + 3] + K - 1;
-                float* t = tile + m * C * cs1;
-#pragma unroll
-                for (int c = 0; c < C; c++)
-#pragma unroll
-                    for (int g = 0; g < 3; g++) {
-                        const int ty = lrow + 4 * g;
-                        if (lcol < aw && ty < ah) t[c * cs1 + ty * L.tp1 + lcol] = pre[m][c][g];
-                    }
-            }
-            wave_sync();
-        };
-        if (nfw > 0) windows(0);
-        if (nbw > 0) windows(1);
-        int pend = -1;  // task whose tiles are in `pre`
-        if (ntask > 0) {
-            int c0, f0;
-            task(0, c0, f0);
-            issue(c0);
-            store(c0);
-            if (ntask > 1) {
-                int c1, f1;
-                task(1, c1, f1);
-                if (c1 != c0) { issue(c1); pend = 1; }
-            }
-        }
-        for (int ti = 0; ti < ntask; ti++) {
-            int ch, f;
-            task(ti, ch, f);
-            // one candidate per lane: slot = m * P2p + l
+        if (nst > 0) windows(0);
+        for (int f = 0; f < nst; f++) {
+            const bool act[2] = {f < nfw, f < nbw};
+            // the areas of all windows of the frame: every load is issued
+            // before the first store; lanes cover 4 rows x 16 columns.  The
+            // empty asm keeps the 36 store addresses from being hoisted out
+            // of the frame loop (they would not fit in registers).
             {
-                const int s = lane, m = s >> lg2, l = s & (P2p - 1);
+                unsigned lofs = (unsigned)(lrow * L.tp1 + lcol);
+                asm volatile("" : "+v"(lofs));
+                float pre[2][2][C][3];  // [chain][window][channel][row group]
+#pragma unroll
+                for (int ch = 0; ch < 2; ch++)
+#pragma unroll
+                    for (int m = 0; m < 2; m++) {
+                        const int* w5 = win + (ch * Nb + m) * 5;
+                        const bool on = act[ch] && m < Nb;
+                        const int aw = on ? w5[2] + K - 1 : 0, ah = on ? w5[3] + K - 1 : 0;
+                        const unsigned fb = on ? (unsigned)w5[4] * whc + (unsigned)w5[1] * W + (unsigned)w5[0] : 0u;
+#pragma unroll
+                        for (int c = 0; c < C; c++)
+#pragma unroll
+                            for (int g = 0; g < 3; g++) {
+                                const int ty = lrow + 4 * g;
+                                pre[ch][m][c][g] = (lcol < aw && ty < ah) ? vid[fb + c * wh + ty * W + lcol] : 0.f;
+                            }
+                    }
+#pragma unroll
+                for (int ch = 0; ch < 2; ch++)
+#pragma unroll
+                    for (int m = 0; m < 2; m++) {
+                        const int* w5 = win + (ch * Nb + m) * 5;
+                        const bool on = act[ch] && m < Nb;
+                        const int aw = on ? w5[2] + K - 1 : 0, ah = on ? w5[3] + K - 1 : 0;
+                        float* t = tile + lofs + (ch * Nb + m) * C * cs1;
+#pragma unroll
+                        for (int c = 0; c < C; c++)
+#pragma unroll
+                            for (int g = 0; g < 3; g++) {
+                                const int ty = lrow + 4 * g;
+                                if (lcol < aw && ty < ah) t[c * cs1 + 4 * g * L.tp1] = pre[ch][m][c][g];
+                            }
+                    }
+            }
+            wave_sync();
+            // slot lane of chain h: window m = lane / P2p, raster index l;
+            // a valid slot keeps its tile offset (bit 31: the reference's
+            // own position) in s_d until its distance replaces it
+            unsigned long long bal[2];
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                const int m = lane >> lg2, l = lane & (P2p - 1);
                 bool valid = false;
-                if (m < Nb) {
-                    const int wn = ch * Nb + m;
+                if (act[h] && m < Nb) {
+                    const int wn = h * Nb + m;
                     const int x0 = win[wn * 5], y0 = win[wn * 5 + 1], wx = win[wn * 5 + 2], wy = win[wn * 5 + 3];
                     if (l < wx * wy) {
                         const int ly = small_div(l, wx), lx = l - ly * wx;
                         const int qx = x0 + lx, qy = y0 + ly;
+                        // already seen = inside an earlier window of this frame
                         bool seen = false;
                         for (int m2 = 0; m2 < m; m2++) {
-                            const int* w2 = win + (ch * Nb + m2) * 5;
+                            const int* w2 = win + (h * Nb + m2) * 5;
                             seen |= qx >= w2[0] && qx < w2[0] + w2[2] && qy >= w2[1] && qy < w2[1] + w2[3];
                         }
                         valid = !seen;
                         if (valid) {
-                            const float sv = ssd1(tile + m * C * cs1 + ly * L.tp1 + lx, L.tp1, cs1);
-                            s_d[s] = sv / norm - (((unsigned)qx == rx && (unsigned)qy == ry) ? a.d : 0.f);
-                            s_i[s] = (unsigned)win[wn * 5 + 4] * whc + qy * W + qx;
+                            const unsigned toff = (unsigned)(wn * C * cs1 + ly * L.tp1 + lx);
+                            const bool self = (unsigned)qx == rx && (unsigned)qy == ry;
+                            s_d[h * 64 + lane] = __uint_as_float(toff | (self ? 0x80000000u : 0u));
+                            s_i[h * 64 + lane] = (unsigned)win[wn * 5 + 4] * whc + qy * W + qx;
                         }
                     }
                 }
-                s_v[s] = valid;
+                bal[h] = __ballot(valid);
+                if (valid) lst[(h ? __popcll(bal[0]) : 0) + lanes_below(bal[h], lane)] = (unsigned)(h * 64 + lane);
             }
             wave_sync();
-            int nfb = 0;
-            for (int m = 0; m < Nb; m++) {
-                const int sb = m * P2p;
-                const bool v = lane < P2p && s_v[sb + lane];
-                const unsigned long long bal = __ballot(v);
-                const int mm = __popcll(bal);
-                n_eval += mm;
-                const int nb = min(Nb, mm);
-                wave_select_sparse(s_d + sb, s_i + sb, v, bal, nb, o_d, o_i, c_d, c_i, scr, lane);
-                if (lane < nb) {
-                    f_d[nfb + lane] = o_d[lane];
-                    f_i[nfb + lane] = o_i[lane];
+            const int nv = __popcll(bal[0]) + __popcll(bal[1]);
+            for (int e0 = 0; e0 < nv; e0 += 64) {
+                if (e0 + lane < nv) {
+                    const unsigned sl = lst[e0 + lane];
+                    const unsigned tb = __float_as_uint(s_d[sl]);
+                    const float sv = ssd(tile + (tb & 0x7FFFFFFFu), L.tp1, cs1);
+                    s_d[sl] = sv / norm - ((tb >> 31) ? a.d : 0.f);
                 }
-                nfb += nb;
+            }
+            wave_sync();
+            for (int ch = 0; ch < 2; ch++) {
+                if (!act[ch]) continue;
+                // top Nb of each window's list, appended in window order
+                int nfb = 0;
+                for (int m = 0; m < Nb; m++) {
+                    const int sb = ch * 64 + m * P2p;
+                    const unsigned long long wmask = P2p >= 64 ? ~0ull : ((1ull << P2p) - 1ull);
+                    const unsigned long long bm = (bal[ch] >> (m * P2p)) & wmask;
+                    const bool v = (bm >> lane) & 1ull;
+                    const int mm = __popcll(bm);
+                    n_eval += mm;
+                    const int nb = min(Nb, mm);
+                    wave_select_sparse(s_d + sb, s_i + sb, v, bm, nb, o_d, o_i, c_d, c_i, scr, lane);
+                    if (lane < nb) {
+                        f_d[nfb + lane] = o_d[lane];
+                        f_i[nfb + lane] = o_i[lane];
+                    }
+                    nfb += nb;
+                    wave_sync();
+                }
+                wave_select(f_d, f_i, nfb, Nb, o_d, o_i, scr, lane);
+                const int at = Nb * (1 + (ch == 0 ? f : nfw + f));
+                if (lane < Nb) {
+                    pp[(ch == 0 ? 8 : 0) + lane] = o_i[lane];
+                    b_d[at + lane] = o_d[lane];
+                    b_i[at + lane] = o_i[lane];
+                }
                 wave_sync();
             }
-            wave_select(f_d, f_i, nfb, Nb, o_d, o_i, scr, lane);
-            const int at = Nb * (1 + (ch == 0 ? f : nfw + f));
-            if (lane < Nb) {
-                pp[(ch == 0 ? 8 : 0) + lane] = o_i[lane];
-                b_d[at + lane] = o_d[lane];
-                b_i[at + lane] = o_i[lane];
-            }
-            wave_sync();
-            if (f + 1 < nfr[ch]) windows(ch);
-            // the next task's tiles into the (now free) tile region
-            if (ti + 1 < ntask) {
-                int cn, fn;
-                task(ti + 1, cn, fn);
-                if (pend == ti + 1) {
-                    store(cn);
-                } else {
-                    issue(cn);
-                    store(cn);
-                }
-                pend = -1;
-                if (ti + 2 < ntask) {
-                    int c2, f2;
-                    task(ti + 2, c2, f2);
-                    if (c2 == ch && cn != ch) { issue(c2); pend = ti + 2; }
-                }
-            }
+            if (f + 1 < nst) windows(f + 1);
         }
     }
     for (int f = 0; f < (piped ? 0 : nst); f++) {
@@ -673,7 +649,7 @@ constexpr int bm_waves_floor() { return C == 3 ? 5 : 1; }
 template <int K, int C>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(bm_waves_floor<K, C>()))) void k_bm(BMArgs a) {
     extern __shared__ float smem[];
-    const BMLay L = bm_layout(K, C, a.Ns, a.Npr, a.Nb);
+    const BMLay L = bm_layout(K, C, a.Ns, a.Npr, a.Nb, a.Nf);
     const int lane = threadIdx.x & 63;
     const int w = threadIdx.x >> 6;
     const int nw = blockDim.x >> 6;
@@ -1620,7 +1596,7 @@ template <int K, int C>
 static hipError_t bm_k(const BMArgs& a, hipStream_t s) {
     const long n = a.r_end - a.r_begin;
     if (n <= 0) return hipSuccess;
-    const BMLay L = bm_layout(K, C, a.Ns, a.Npr, a.Nb);
+    const BMLay L = bm_layout(K, C, a.Ns, a.Npr, a.Nb, a.Nf);
     const size_t per_wave = (size_t)L.total * 4;
     int waves = 4;
     while (waves > 1 && per_wave * waves > 64 * 1024) waves--;
@@ -1825,7 +1801,7 @@ hipError_t launch_zero_u64(unsigned long long* p, int n, hipStream_t s) {
     return hipGetLastError();
 }
 
-int bm_lds_words(int K, int C, int Ns, int Npr, int Nb) { return bm_layout(K, C, Ns, Npr, Nb).total; }
+int bm_lds_words(int K, int C, int Ns, int Npr, int Nb, int Nf) { return bm_layout(K, C, Ns, Npr, Nb, Nf).total; }
 int filter_words(int K, int C, int N, int step) { return filter_lds_words(K, C, N, step); }
 
 hipError_t launch_aggregate(int C, const AArgs& a, unsigned frames, hipStream_t s) {
