@@ -188,6 +188,21 @@ __device__ void wave_select_sparse(const float* ld, const uint32_t* li, bool val
     wave_sync();
 }
 
+typedef __attribute__((address_space(3))) void lds_void;
+
+// LDS byte address of a pointer into the wave's LDS, made wave-uniform.
+__device__ __forceinline__ unsigned lds_addr(const float* p) {
+    return __builtin_amdgcn_readfirstlane((unsigned)(size_t)(const lds_void*)p);
+}
+
+// Copies one 4-byte word per active lane from src (per lane) to the LDS
+// byte address lds + 4*lane (lds wave-uniform) without staging it in
+// registers (global_load_lds_dword).  The caller waits for vmcnt before
+// reading it.
+__device__ __forceinline__ void lds_dma(const void* src, unsigned lds) {
+    __builtin_amdgcn_global_load_lds(src, (lds_void*)(size_t)lds, 4, 0, 0);
+}
+
 // ---------------------------------------------------------------------
 // Predictive block matching: computeSimilarPatches / localSearch /
 // patchDistance (src/vbm3d.cpp:930-1197).
@@ -377,64 +392,71 @@ __device__ __forceinline__ void bm_one(const A& a, const long r, float* base, co
             }
             wave_sync();
         };
+        // a (window, channel) area is th1 rows of pitch tp1 (at most 192
+        // words); word e = k*64 + lane sits at row e / tp1, column e % tp1
+        int er[3], ec[3];
+        unsigned goff[3];  // byte offset in the frame
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            er[k] = small_div(k * 64 + lane, L.tp1);
+            ec[k] = k * 64 + lane - er[k] * L.tp1;
+            goff[k] = ((unsigned)er[k] * W + (unsigned)ec[k]) * 4u;
+        }
+        const unsigned tile_lds = lds_addr(tile);
+        // slot lane of a chain: window m = lane / P2p, position l = ly*Npr + lx
+        // of the Npr x Npr grid (the valid positions of a clipped window
+        // keep its raster order)
+        const int sm = lane >> lg2, sl = lane & (P2p - 1);
+        const int sly = small_div(sl, a.Npr), slx = sl - sly * a.Npr;
         if (nst > 0) windows(0);
         for (int f = 0; f < nst; f++) {
             const bool act[2] = {f < nfw, f < nbw};
-            // the areas of all windows of the frame: every load is issued
-            // before the first store; lanes cover 4 rows x 16 columns.  The
-            // empty asm keeps the 36 store addresses from being hoisted out
-            // of the frame loop (they would not fit in registers).
+            // the areas of all windows of the frame, from memory straight
+            // into the tile: a wave-uniform row base plus the lane's byte
+            // offset.  The empty asm keeps the LDS addresses from being
+            // hoisted out of the frame loop.
             {
-                unsigned lofs = (unsigned)(lrow * L.tp1 + lcol);
-                asm volatile("" : "+v"(lofs));
-                float pre[2][2][C][3];  // [chain][window][channel][row group]
+                const int ne = L.th1 * L.tp1;
+                unsigned tl = tile_lds;
+                asm volatile("" : "+s"(tl));
 #pragma unroll
                 for (int ch = 0; ch < 2; ch++)
 #pragma unroll
                     for (int m = 0; m < 2; m++) {
-                        const int* w5 = win + (ch * Nb + m) * 5;
-                        const bool on = act[ch] && m < Nb;
-                        const int aw = on ? w5[2] + K - 1 : 0, ah = on ? w5[3] + K - 1 : 0;
-                        const unsigned fb = on ? (unsigned)w5[4] * whc + (unsigned)w5[1] * W + (unsigned)w5[0] : 0u;
+                        if (!act[ch] || m >= Nb) continue;
+                        const int wn = ch * Nb + m;
+                        const int aw = __builtin_amdgcn_readfirstlane(win[wn * 5 + 2]) + K - 1;
+                        const int ah = __builtin_amdgcn_readfirstlane(win[wn * 5 + 3]) + K - 1;
+                        const unsigned fb = (unsigned)__builtin_amdgcn_readfirstlane(win[wn * 5 + 4]) * whc +
+                                            (unsigned)__builtin_amdgcn_readfirstlane(win[wn * 5 + 1]) * W +
+                                            (unsigned)__builtin_amdgcn_readfirstlane(win[wn * 5]);
 #pragma unroll
-                        for (int c = 0; c < C; c++)
+                        for (int k = 0; k < 3; k++) {
+                            if (k * 64 >= ne) break;
+                            if (er[k] < ah && ec[k] < aw) {
 #pragma unroll
-                            for (int g = 0; g < 3; g++) {
-                                const int ty = lrow + 4 * g;
-                                pre[ch][m][c][g] = (lcol < aw && ty < ah) ? vid[fb + c * wh + ty * W + lcol] : 0.f;
+                                for (int c = 0; c < C; c++)
+                                    lds_dma((const char*)(vid + fb + c * wh) + goff[k],
+                                            tl + (unsigned)(((wn * C + c) * cs1 + k * 64) * 4));
                             }
+                        }
                     }
-#pragma unroll
-                for (int ch = 0; ch < 2; ch++)
-#pragma unroll
-                    for (int m = 0; m < 2; m++) {
-                        const int* w5 = win + (ch * Nb + m) * 5;
-                        const bool on = act[ch] && m < Nb;
-                        const int aw = on ? w5[2] + K - 1 : 0, ah = on ? w5[3] + K - 1 : 0;
-                        float* t = tile + lofs + (ch * Nb + m) * C * cs1;
-#pragma unroll
-                        for (int c = 0; c < C; c++)
-#pragma unroll
-                            for (int g = 0; g < 3; g++) {
-                                const int ty = lrow + 4 * g;
-                                if (lcol < aw && ty < ah) t[c * cs1 + 4 * g * L.tp1] = pre[ch][m][c][g];
-                            }
-                    }
+                __builtin_amdgcn_s_waitcnt(0);
             }
             wave_sync();
-            // slot lane of chain h: window m = lane / P2p, raster index l;
-            // a valid slot keeps its tile offset (bit 31: the reference's
-            // own position) in s_d until its distance replaces it
+            // slot lane of chain h; a valid slot keeps its tile offset (bit
+            // 31: the reference's own position) in s_d until its distance
+            // replaces it
             unsigned long long bal[2];
 #pragma unroll
             for (int h = 0; h < 2; h++) {
-                const int m = lane >> lg2, l = lane & (P2p - 1);
+                const int m = sm;
                 bool valid = false;
-                if (act[h] && m < Nb) {
+                if (act[h] && m < Nb && sl < P2) {
                     const int wn = h * Nb + m;
                     const int x0 = win[wn * 5], y0 = win[wn * 5 + 1], wx = win[wn * 5 + 2], wy = win[wn * 5 + 3];
-                    if (l < wx * wy) {
-                        const int ly = small_div(l, wx), lx = l - ly * wx;
+                    if (slx < wx && sly < wy) {
+                        const int ly = sly, lx = slx;
                         const int qx = x0 + lx, qy = y0 + ly;
                         // already seen = inside an earlier window of this frame
                         bool seen = false;
