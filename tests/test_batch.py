@@ -107,6 +107,7 @@ def test_autotune_leaves_inputs_and_matches(env, reference):
     print("autotuned:", best)
     for it, nz in zip(items, clips):
         assert np.array_equal(it.noisy.cpu().numpy().view(np.uint32), nz.view(np.uint32))
-    assert 1 <= best["waves_per_group"] <= 4 and best["grab"] >= 1 and best["groups_per_cu"] >= 1
+    # 0 = as many waves per group as fit (the default, kept when no trial beats it)
+    assert 0 <= best["waves_per_group"] <= 4 and best["grab"] >= 1 and best["groups_per_cu"] >= 1
     V.run_batch(items)  # the tuned configuration, with a graph
     _same(items, reference, "autotuned")

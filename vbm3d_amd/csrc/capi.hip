@@ -533,6 +533,7 @@ struct BatchOp {
     unsigned W = 0, H = 0, F = 0;
     int fwd = 0;
     size_t bytes = 0;
+    int shape = 0;  // match: bm_shape shared by every job, else 0
 };
 
 struct BatchPlan {
@@ -827,6 +828,9 @@ int plan_batch(vbm3d_ctx* ctx, const vbm3d_clip* clips, int n, int nb, const vbm
                     op.jobs = blob_put(plan.blob, cl.bm.data(), cl.bm.size() * sizeof(BMArgs));
                     op.prefix = blob_put(plan.blob, cl.pre.data(), cl.pre.size() * sizeof(long));
                     op.njobs = (int)cl.bm.size(); op.total = cl.pre.back(); op.words = cl.wb;
+                    op.shape = 1;
+                    for (const BMArgs& b : cl.bm)
+                        if (bm_shape(b) != 1) op.shape = 0;
                     op.counter = plan.counters++;
                     plan.ops.push_back(op);
                 }
@@ -900,12 +904,13 @@ int exec_batch(vbm3d_ctx* ctx, const BatchPlan& plan, const vbm3d_batch_config& 
             const long blocks = std::min<long>(need, (long)cfg.groups_per_cu * cus);
             if (!cfg.persistent) {
                 if (op.kind == 0)
-                    HIP_TRY(launch_bm_multi(op.K, op.C, (const BMArgs*)(blob + op.jobs), q, op.words, waves, s));
+                    HIP_TRY(launch_bm_multi(op.K, op.C, (const BMArgs*)(blob + op.jobs), q, op.words, waves, op.shape, s));
                 else
                     HIP_TRY(launch_filter_multi(op.K, op.C, op.step, (const FArgs*)(blob + op.jobs), q, op.words,
                                                 waves, s));
             } else if (op.kind == 0) {
-                HIP_TRY(launch_bm_queue(op.K, op.C, (const BMArgs*)(blob + op.jobs), q, op.words, waves, (int)blocks, s));
+                HIP_TRY(launch_bm_queue(op.K, op.C, (const BMArgs*)(blob + op.jobs), q, op.words, waves, (int)blocks,
+                                            op.shape, s));
             } else {
                 HIP_TRY(launch_filter_queue(op.K, op.C, op.step, (const FArgs*)(blob + op.jobs), q, op.words, waves,
                                             (int)blocks, s));

@@ -86,14 +86,14 @@ __host__ __device__ inline BMLay bm_layout(int K, int C, int Ns, int Npr, int Nb
     L.sv = o; o += L.piped ? 0 : S;
     L.cd = o; o += 64;
     L.ci = o; o += 64;  // directly after cd: the paired form's slot list spans both
-    L.bd = o; o += nbest;
-    L.bi = o; o += nbest;
     L.fd = o; o += L.piped ? 8 : 64;
     L.fi = o; o += L.piped ? 8 : 64;
     L.od = o; o += 16;
     L.oi = o; o += 16;
     L.pp = o; o += 16;
     L.win = o; o += L.piped ? 20 : 80;
+    L.bd = o; o += nbest;  // last: the only arrays whose place depends on Nf
+    L.bi = o; o += nbest;
     L.total = o;
     return L;
 }
@@ -170,6 +170,10 @@ struct WorkQueue {
 };
 
 // Launchers (kernels.hip).  Return hipSuccess or the launch error.
+// 1: the search shape the kernels also hold as compile-time constants
+// (Ns 7, Npr 5, Nb 2: the default parameters of both steps); 0 otherwise.
+inline int bm_shape(const BMArgs& a) { return a.Ns == 7 && a.Npr == 5 && a.Nb == 2 ? 1 : 0; }
+
 hipError_t launch_bm(int K, const BMArgs& a, hipStream_t s);
 hipError_t launch_filter(int K, int step, const FArgs& a, hipStream_t s);
 hipError_t launch_aggregate(int C, const AArgs& a, unsigned frames, hipStream_t s);
@@ -177,11 +181,11 @@ hipError_t launch_color(float* v, unsigned W, unsigned H, unsigned F, bool forwa
                         hipStream_t s);
 hipError_t launch_filter_adjoint(int K, const FArgs& a, hipStream_t s);
 hipError_t launch_bm_queue(int K, int C, const BMArgs* jobs, const WorkQueue& q, int words, int waves,
-                           int blocks, hipStream_t s);
+                           int blocks, int shape, hipStream_t s);
 hipError_t launch_filter_queue(int K, int C, int step, const FArgs* jobs, const WorkQueue& q, int words,
                                int waves, int blocks, hipStream_t s);
 hipError_t launch_bm_multi(int K, int C, const BMArgs* jobs, const WorkQueue& q, int words, int waves,
-                           hipStream_t s);
+                           int shape, hipStream_t s);
 hipError_t launch_filter_multi(int K, int C, int step, const FArgs* jobs, const WorkQueue& q, int words,
                                int waves, hipStream_t s);
 hipError_t launch_aggregate_queue(int C, const AArgs* jobs, const WorkQueue& q, int blocks, hipStream_t s);

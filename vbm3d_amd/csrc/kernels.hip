@@ -213,9 +213,13 @@ __device__ __forceinline__ void lds_dma(const void* src, unsigned lds) {
 // one candidate's sum of squared differences from LDS against the
 // reference patch (also in LDS), in the reference's (c, y, x) order.
 // ---------------------------------------------------------------------
-template <int K, int C, class A>
+template <int K, int C, int SH, class A>
 __device__ __forceinline__ void bm_one(const A& a, const long r, float* base, const int lane) {
-    const BMLay L = bm_layout(K, C, a.Ns, a.Npr, a.Nb, a.Nf);
+    // SH = 1: the default search shape (Ns 7, Npr 5, Nb 2) fixed at compile
+    // time, so that tile and window offsets become instruction immediates
+    constexpr bool FIX = SH == 1;
+    const int Ns = FIX ? 7 : a.Ns, Npr = FIX ? 5 : a.Npr, Nb = FIX ? 2 : a.Nb;
+    const BMLay L = bm_layout(K, C, Ns, Npr, Nb, a.Nf);
     float* tile = base + L.tile;
     float* s_d = base + L.sd;
     uint32_t* s_i = (uint32_t*)(base + L.si);
@@ -314,7 +318,7 @@ __device__ __forceinline__ void bm_one(const A& a, const long r, float* base, co
     int nbest;
     unsigned n_eval = 0;
     {
-        const int h = (a.Ns - 1) / 2;
+        const int h = (Ns - 1) / 2;
         const int x0 = max(0, (int)rx - h), y0 = max(0, (int)ry - h);
         const int x1 = min((int)W - K, (int)rx + h), y1 = min((int)H - K, (int)ry + h);
         const int wx = x1 - x0 + 1, wy = y1 - y0 + 1, m = wx * wy;
@@ -330,7 +334,7 @@ __device__ __forceinline__ void bm_one(const A& a, const long r, float* base, co
         }
         wave_sync();
         n_eval += m;
-        const int nb = min(a.Nb, m);
+        const int nb = min(Nb, m);
         wave_select(c_d, c_i, m, nb, o_d, o_i, scr, lane);
         if (lane < nb) {
             b_d[lane] = o_d[lane];
@@ -347,13 +351,12 @@ __device__ __forceinline__ void bm_one(const A& a, const long r, float* base, co
     //    backward frame f are searched in the same pass; their picks are
     //    stored where the reference's sequential order puts them (all
     //    forward frames, then all backward frames).
-    const int P2 = a.Npr * a.Npr, hs = (a.Npr - 1) / 2;
+    const int P2 = Npr * Npr, hs = (Npr - 1) / 2;
     const int P2p = pow2ceil(P2), lg2 = __builtin_ctz(P2p);
     const int cs1 = L.tp1 * L.th1;
     const int nfw = (int)(min(rt + (unsigned)a.Nf, a.F - (unsigned)a.kt) - rt);
     const int nbw = (int)rt - max((int)rt - a.Nf, 0);
     const int nst = max(nfw, nbw);
-    const int Nb = a.Nb;
     // Paired form (at most 2 matches per frame, windows of at most 64
     // slots per chain, at most 12 tile rows): frame f of the forward chain
     // and frame f of the backward chain are searched together.  Chain h
@@ -407,7 +410,7 @@ __device__ __forceinline__ void bm_one(const A& a, const long r, float* base, co
         // of the Npr x Npr grid (the valid positions of a clipped window
         // keep its raster order)
         const int sm = lane >> lg2, sl = lane & (P2p - 1);
-        const int sly = small_div(sl, a.Npr), slx = sl - sly * a.Npr;
+        const int sly = small_div(sl, Npr), slx = sl - sly * Npr;
         if (nst > 0) windows(0);
         for (int f = 0; f < nst; f++) {
             const bool act[2] = {f < nfw, f < nbw};
@@ -668,19 +671,19 @@ __device__ __forceinline__ void bm_one(const A& a, const long r, float* base, co
 template <int K, int C>
 constexpr int bm_waves_floor() { return C == 3 ? 5 : 1; }
 
-template <int K, int C>
+template <int K, int C, int SH>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(bm_waves_floor<K, C>()))) void k_bm(BMArgs a) {
     extern __shared__ float smem[];
-    const BMLay L = bm_layout(K, C, a.Ns, a.Npr, a.Nb, a.Nf);
+    const BMLay L = SH == 1 ? bm_layout(K, C, 7, 5, 2, a.Nf) : bm_layout(K, C, a.Ns, a.Npr, a.Nb, a.Nf);
     const int lane = threadIdx.x & 63;
     const int w = threadIdx.x >> 6;
     const int nw = blockDim.x >> 6;
     const long r = a.r_begin + (long)blockIdx.x * nw + w;
     if (r >= a.r_end) return;  // wave-uniform
-    bm_one<K, C>(a, r, smem + (size_t)w * L.total, lane);
+    bm_one<K, C, SH>(a, r, smem + (size_t)w * L.total, lane);
 }
 
-template <int K, int C>
+template <int K, int C, int SH>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(bm_waves_floor<K, C>()))) void k_bm_multi(const BMArgs* __restrict__ jobs, WorkQueue q, int words) {
     extern __shared__ float smem[];
     const int lane = threadIdx.x & 63;
@@ -689,12 +692,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(bm_waves_fl
     const int j = queue_job(q, g);
     typedef const __attribute__((address_space(4))) BMArgs cBM;
     cBM& a = ((cBM*)jobs)[j];
-    bm_one<K, C>(a, a.r_begin + (g - uniform_long(q.item_begin[j])), smem + (size_t)(threadIdx.x >> 6) * words, lane);
+    bm_one<K, C, SH>(a, a.r_begin + (g - uniform_long(q.item_begin[j])), smem + (size_t)(threadIdx.x >> 6) * words, lane);
 }
 
 // Persistent form over a batch: every wave takes `grab` items at a time
 // from the queue counter until the queue is empty; item -> (job, patch).
-template <int K, int C>
+template <int K, int C, int SH>
 __global__ __launch_bounds__(256) void k_bm_queue(const BMArgs* __restrict__ jobs, WorkQueue q, int words) {
     extern __shared__ float smem[];
     const int lane = threadIdx.x & 63;
@@ -709,7 +712,7 @@ __global__ __launch_bounds__(256) void k_bm_queue(const BMArgs* __restrict__ job
         for (long it = it0; it < it1; it++) {
             queue_seek(q, it, j, jb, je);
             cBM& a = ((cBM*)jobs)[j];  // scalar loads of the job
-            bm_one<K, C>(a, a.r_begin + (it - jb), base, lane);
+            bm_one<K, C, SH>(a, a.r_begin + (it - jb), base, lane);
         }
         it0 = queue_value(nxt);
     }
@@ -1623,8 +1626,9 @@ static hipError_t bm_k(const BMArgs& a, hipStream_t s) {
     int waves = 4;
     while (waves > 1 && per_wave * waves > 64 * 1024) waves--;
     if (per_wave > 64 * 1024) return hipErrorInvalidValue;
-    hipLaunchKernelGGL((k_bm<K, C>), dim3((unsigned)((n + waves - 1) / waves)), dim3(64 * waves),
-                       per_wave * waves, s, a);
+    const dim3 grid((unsigned)((n + waves - 1) / waves)), block(64 * waves);
+    if (bm_shape(a) == 1) hipLaunchKernelGGL((k_bm<K, C, 1>), grid, block, per_wave * waves, s, a);
+    else hipLaunchKernelGGL((k_bm<K, C, 0>), grid, block, per_wave * waves, s, a);
     return hipGetLastError();
 }
 
@@ -1702,28 +1706,32 @@ hipError_t launch_adjoint_fallback(float* gz, const float* gy, const float* den,
 
 // ---- persistent launches over a batch (jobs and queue arrays in device memory)
 template <int K, int C>
-static hipError_t bm_queue_k(const BMArgs* jobs, const WorkQueue& q, int words, int waves, int blocks,
+static hipError_t bm_queue_k(const BMArgs* jobs, const WorkQueue& q, int words, int waves, int blocks, int shape,
                              hipStream_t s) {
-    hipLaunchKernelGGL((k_bm_queue<K, C>), dim3(blocks), dim3(64 * waves), (size_t)words * 4 * waves, s, jobs, q,
-                       words);
+    const size_t lds = (size_t)words * 4 * waves;
+    if (shape == 1) hipLaunchKernelGGL((k_bm_queue<K, C, 1>), dim3(blocks), dim3(64 * waves), lds, s, jobs, q, words);
+    else hipLaunchKernelGGL((k_bm_queue<K, C, 0>), dim3(blocks), dim3(64 * waves), lds, s, jobs, q, words);
     return hipGetLastError();
 }
 
 template <int K, int C>
-static hipError_t bm_multi_k(const BMArgs* jobs, const WorkQueue& q, int words, int waves, hipStream_t s) {
-    const long blocks = (q.total + waves - 1) / waves;
-    hipLaunchKernelGGL((k_bm_multi<K, C>), dim3((unsigned)blocks), dim3(64 * waves), (size_t)words * 4 * waves, s,
-                       jobs, q, words);
+static hipError_t bm_multi_k(const BMArgs* jobs, const WorkQueue& q, int words, int waves, int shape,
+                             hipStream_t s) {
+    const dim3 grid((unsigned)((q.total + waves - 1) / waves)), block(64 * waves);
+    const size_t lds = (size_t)words * 4 * waves;
+    if (shape == 1) hipLaunchKernelGGL((k_bm_multi<K, C, 1>), grid, block, lds, s, jobs, q, words);
+    else hipLaunchKernelGGL((k_bm_multi<K, C, 0>), grid, block, lds, s, jobs, q, words);
     return hipGetLastError();
 }
 
-hipError_t launch_bm_multi(int K, int C, const BMArgs* jobs, const WorkQueue& q, int words, int waves,
+hipError_t launch_bm_multi(int K, int C, const BMArgs* jobs, const WorkQueue& q, int words, int waves, int shape,
                            hipStream_t s) {
     if (q.total <= 0) return hipSuccess;
     if ((size_t)words * 4 * waves > 64 * 1024 || waves < 1 || waves > 4) return hipErrorInvalidValue;
 #define VB_BMM(KK)                                                                               \
     case KK:                                                                                     \
-        return C == 1 ? bm_multi_k<KK, 1>(jobs, q, words, waves, s) : bm_multi_k<KK, 3>(jobs, q, words, waves, s);
+        return C == 1 ? bm_multi_k<KK, 1>(jobs, q, words, waves, shape, s)                       \
+                      : bm_multi_k<KK, 3>(jobs, q, words, waves, shape, s);
     switch (K) {
         VB_BMM(4) VB_BMM(5) VB_BMM(6) VB_BMM(7) VB_BMM(8)
         default: return hipErrorInvalidValue;
@@ -1732,13 +1740,13 @@ hipError_t launch_bm_multi(int K, int C, const BMArgs* jobs, const WorkQueue& q,
 }
 
 hipError_t launch_bm_queue(int K, int C, const BMArgs* jobs, const WorkQueue& q, int words, int waves,
-                           int blocks, hipStream_t s) {
+                           int blocks, int shape, hipStream_t s) {
     if (q.total <= 0 || blocks <= 0) return hipSuccess;
     if ((size_t)words * 4 * waves > 64 * 1024 || waves < 1 || waves > 4) return hipErrorInvalidValue;
 #define VB_BMQ(KK)                                                                               \
     case KK:                                                                                     \
-        return C == 1 ? bm_queue_k<KK, 1>(jobs, q, words, waves, blocks, s)                      \
-                      : bm_queue_k<KK, 3>(jobs, q, words, waves, blocks, s);
+        return C == 1 ? bm_queue_k<KK, 1>(jobs, q, words, waves, blocks, shape, s)               \
+                      : bm_queue_k<KK, 3>(jobs, q, words, waves, blocks, shape, s);
     switch (K) {
         VB_BMQ(4) VB_BMQ(5) VB_BMQ(6) VB_BMQ(7) VB_BMQ(8)
         default: return hipErrorInvalidValue;
