@@ -533,7 +533,7 @@ struct BatchOp {
     unsigned W = 0, H = 0, F = 0;
     int fwd = 0;
     size_t bytes = 0;
-    int shape = 0;  // match: bm_shape shared by every job, else 0
+    int shape = 0;  // match: bm_shape shared by every job, else 0; aggregate: largest N
 };
 
 struct BatchPlan {
@@ -849,6 +849,8 @@ int plan_batch(vbm3d_ctx* ctx, const vbm3d_clip* clips, int n, int nb, const vbm
                     if (v.empty()) continue;
                     BatchOp op;
                     op.kind = 2; op.C = cc; op.step = st;
+                    op.shape = 1;  // largest group size of the jobs
+                    for (const AArgs& g : v) op.shape = std::max(op.shape, g.N);
                     op.jobs = blob_put(plan.blob, v.data(), v.size() * sizeof(AArgs));
                     op.prefix = blob_put(plan.blob, pre.data(), pre.size() * sizeof(long));
                     op.njobs = (int)v.size(); op.total = pre.back();
@@ -894,7 +896,7 @@ int exec_batch(vbm3d_ctx* ctx, const BatchPlan& plan, const vbm3d_batch_config& 
             q.grab = op.kind == 2 ? 1 : std::max(1, cfg.grab);
             if (op.kind == 2) {
                 const long blocks = std::min<long>(op.total, (long)cfg.groups_per_cu * cus);
-                HIP_TRY(launch_aggregate_queue(op.C, (const AArgs*)(blob + op.jobs), q, (int)blocks, s));
+                HIP_TRY(launch_aggregate_queue(op.C, (const AArgs*)(blob + op.jobs), q, (int)blocks, op.shape, s));
                 continue;
             }
             int waves = cfg.waves_per_group > 0 ? std::min(cfg.waves_per_group, 4) : 4;

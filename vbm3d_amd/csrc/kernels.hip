@@ -1380,11 +1380,14 @@ __device__ __forceinline__ int lower_bound_u16(const uint16_t* a, int n, int v) 
 // in list order: every output sample sees its contributions in exactly the
 // reference's order.
 template <int C, bool RAW, class A>
-__device__ __forceinline__ void agg_tile(const A& a, const int bx, const int by, const unsigned bz) {
+__device__ __forceinline__ void agg_tile(const A& a, const int bx, const int by, const unsigned bz, const int nl) {
+    typedef const __attribute__((address_space(4))) float cfloat;
     constexpr int TS = kAggTile;
     constexpr int QS = TS / 2;
-    __shared__ uint32_t L_sn[256 * kMaxN];
-    __shared__ uint32_t L_pos[256 * kMaxN];
+    // the ordered list of one 256-patch chunk: at most nl >= N members each
+    extern __shared__ uint32_t agg_lists[];
+    uint32_t* L_sn = agg_lists;
+    uint32_t* L_pos = agg_lists + 256 * nl;
     __shared__ float s_kai[kMaxWin * kMaxWin * 4];
     __shared__ int s_w[4];
     const int tid = threadIdx.x;
@@ -1473,23 +1476,53 @@ __device__ __forceinline__ void agg_tile(const A& a, const int bx, const int by,
                 }
                 uint64_t m = __ballot(ov);
                 while (m) {
-                    const int j = j0 + (int)__builtin_ctzll(m);
-                    m &= m - 1;
-                    const uint32_t pos = L_pos[j];
-                    const int dx = px - (int)(pos & 0xFFFFu), dy = py - (int)(pos >> 16);
-                    if (inside && (unsigned)dx < (unsigned)K && (unsigned)dy < (unsigned)K) {
-                        const uint32_t sn = L_sn[j];
+                    // up to 4 entries at a time: every load is issued before
+                    // the first sum, and the sums then run in list order.
+                    // The entry is the same for every lane: its position,
+                    // slot and member are decoded on the scalar unit, its
+                    // weights arrive through scalar loads.
+                    constexpr int U = 4;
+                    int jj[U];
+#pragma unroll
+                    for (int u = 0; u < U; u++) {
+                        jj[u] = m ? j0 + (int)__builtin_ctzll(m) : -1;
+                        m &= m - 1;
+                    }
+                    float ev[U][C], kai[U], w[U][C];
+                    bool on[U];
+#pragma unroll
+                    for (int u = 0; u < U; u++) {
+                        on[u] = false;
+                        kai[u] = 0.f;
+#pragma unroll
+                        for (int c = 0; c < C; c++) ev[u][c] = w[u][c] = 0.f;
+                        if (jj[u] < 0) continue;
+                        const uint32_t pos = __builtin_amdgcn_readfirstlane(L_pos[jj[u]]);
+                        const uint32_t sn = __builtin_amdgcn_readfirstlane(L_sn[jj[u]]);
                         const uint32_t sl = sn / kMaxN, n = sn % kMaxN;
-                        const int kk = dy * K + dx;
-                        const float kai = s_kai[kk];
+                        const cfloat* wb = (const cfloat*)(a.wgt + (size_t)sl * C);
+#pragma unroll
+                        for (int c = 0; c < C; c++) w[u][c] = RAW ? 0.f : wb[c];
+                        const int dx = px - (int)(pos & 0xFFFFu), dy = py - (int)(pos >> 16);
+                        on[u] = inside && (unsigned)dx < (unsigned)K && (unsigned)dy < (unsigned)K;
+                        if (on[u]) {
+                            const int kk = dy * K + dx;
+                            kai[u] = s_kai[kk];
+                            const float* eb = a.est + ((size_t)sl * C * N + n) * K2 + (unsigned)kk;
+#pragma unroll
+                            for (int c = 0; c < C; c++) ev[u][c] = eb[(size_t)c * N * K2];
+                        }
+                    }
+#pragma unroll
+                    for (int u = 0; u < U; u++) {
+                        if (!on[u]) continue;
 #pragma unroll
                         for (int c = 0; c < C; c++) {
-                            const float ev = a.est[(((size_t)sl * C + c) * N + n) * K2 + kk];
                             if (RAW) {
-                                pn[c] = pn[c] + ev;
+                                pn[c] = pn[c] + ev[u][c];
                             } else {
-                                const float kw = kai * a.wgt[(size_t)sl * C + c];
-                                pn[c] = pn[c] + kw * ev;
+                                const float kw = kai[u] * w[u][c];
+                                pn[c] = pn[c] + kw * ev[u][c];
                                 pd[c] = pd[c] + kw;
                             }
                         }
@@ -1522,13 +1555,13 @@ __device__ __forceinline__ void agg_tile(const A& a, const int bx, const int by,
 
 template <int C, bool RAW>
 __global__ __launch_bounds__(256) void k_aggregate(AArgs a) {
-    agg_tile<C, RAW>(a, blockIdx.x, blockIdx.y, blockIdx.z);
+    agg_tile<C, RAW>(a, blockIdx.x, blockIdx.y, blockIdx.z, a.N);
 }
 
 // Persistent form: one tile per take (the workgroup is the unit); item ->
 // (job, frame, tile row, tile column).
 template <int C>
-__global__ __launch_bounds__(256) void k_aggregate_queue(const AArgs* __restrict__ jobs, WorkQueue q) {
+__global__ __launch_bounds__(256) void k_aggregate_queue(const AArgs* __restrict__ jobs, WorkQueue q, int nl) {
     __shared__ long s_item;
     for (;;) {
         __syncthreads();  // every thread has read the previous item
@@ -1542,7 +1575,7 @@ __global__ __launch_bounds__(256) void k_aggregate_queue(const AArgs* __restrict
         const long local = it - q.item_begin[j];
         const long tx = (a.W + kAggTile - 1) / kAggTile, ty = (a.H + kAggTile - 1) / kAggTile;
         const long bz = local / (tx * ty), rem = local - bz * tx * ty;
-        agg_tile<C, false>(a, (int)(rem % tx), (int)(rem / tx), (unsigned)bz);
+        agg_tile<C, false>(a, (int)(rem % tx), (int)(rem / tx), (unsigned)bz, nl);
     }
 }
 
@@ -1818,10 +1851,12 @@ hipError_t launch_filter_queue(int K, int C, int step, const FArgs* jobs, const 
     }
 }
 
-hipError_t launch_aggregate_queue(int C, const AArgs* jobs, const WorkQueue& q, int blocks, hipStream_t s) {
+hipError_t launch_aggregate_queue(int C, const AArgs* jobs, const WorkQueue& q, int blocks, int maxN, hipStream_t s) {
     if (q.total <= 0 || blocks <= 0) return hipSuccess;
-    if (C == 1) hipLaunchKernelGGL(k_aggregate_queue<1>, dim3(blocks), dim3(256), 0, s, jobs, q);
-    else hipLaunchKernelGGL(k_aggregate_queue<3>, dim3(blocks), dim3(256), 0, s, jobs, q);
+    if (maxN < 1 || maxN > kMaxN) return hipErrorInvalidValue;
+    const size_t lds = (size_t)2 * 256 * maxN * sizeof(uint32_t);
+    if (C == 1) hipLaunchKernelGGL(k_aggregate_queue<1>, dim3(blocks), dim3(256), lds, s, jobs, q, maxN);
+    else hipLaunchKernelGGL(k_aggregate_queue<3>, dim3(blocks), dim3(256), lds, s, jobs, q, maxN);
     return hipGetLastError();
 }
 
@@ -1837,12 +1872,14 @@ int filter_words(int K, int C, int N, int step) { return filter_lds_words(K, C, 
 hipError_t launch_aggregate(int C, const AArgs& a, unsigned frames, hipStream_t s) {
     if (frames == 0) return hipSuccess;
     dim3 grid((a.W + kAggTile - 1) / kAggTile, (a.H + kAggTile - 1) / kAggTile, frames);
+    if (a.N < 1 || a.N > kMaxN) return hipErrorInvalidValue;
+    const size_t lds = (size_t)2 * 256 * a.N * sizeof(uint32_t);  // the chunk's ordered list
     if (a.raw) {
-        if (C == 1) hipLaunchKernelGGL((k_aggregate<1, true>), grid, dim3(256), 0, s, a);
-        else hipLaunchKernelGGL((k_aggregate<3, true>), grid, dim3(256), 0, s, a);
+        if (C == 1) hipLaunchKernelGGL((k_aggregate<1, true>), grid, dim3(256), lds, s, a);
+        else hipLaunchKernelGGL((k_aggregate<3, true>), grid, dim3(256), lds, s, a);
     } else {
-        if (C == 1) hipLaunchKernelGGL((k_aggregate<1, false>), grid, dim3(256), 0, s, a);
-        else hipLaunchKernelGGL((k_aggregate<3, false>), grid, dim3(256), 0, s, a);
+        if (C == 1) hipLaunchKernelGGL((k_aggregate<1, false>), grid, dim3(256), lds, s, a);
+        else hipLaunchKernelGGL((k_aggregate<3, false>), grid, dim3(256), lds, s, a);
     }
     return hipGetLastError();
 }
