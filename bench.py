@@ -202,7 +202,10 @@ def measure_batched(api, synth, torch, args):
 
 
 def kernel_name(prm, C):
-    return f"k_bm<{prm.k}, {C}>"
+    # the third template argument is 1 for the search shape compiled with
+    # fixed sizes (Ns 7, Npr 5, Nb 2), as the library's launcher picks it
+    shape = 1 if (prm.Ns, prm.Npr, prm.Nb) == (7, 5, 2) else 0
+    return f"k_bm<{prm.k}, {C}, {shape}>"
 
 
 def load_traffic(workload: str, kernel: str):
