@@ -91,16 +91,19 @@ __device__ __forceinline__ unsigned fkey(float f) {
 // Wave-wide reductions on the DPP network: quad swaps, half-row and row
 // mirrors, then the row-15 / lane-31 broadcasts; lane 63 ends with the
 // result, which readlane makes wave-uniform.
-#define VB_DPP(v, ctrl, rmask) __builtin_amdgcn_update_dpp((int)(v), (int)(v), ctrl, rmask, 0xF, false)
+// The "old" operand is the operation's identity (all ones for an unsigned
+// min, 0 for a sum): lanes a step does not write keep their value, and the
+// compiler folds each move into the min or add as a DPP operand.
+#define VB_DPPMIN(v, ctrl, rmask) __builtin_amdgcn_update_dpp((int)0xFFFFFFFF, (int)(v), ctrl, rmask, 0xF, false)
 #define VB_DPP0(v, ctrl, rmask) __builtin_amdgcn_update_dpp(0, (int)(v), ctrl, rmask, 0xF, false)
 
 __device__ __forceinline__ unsigned wave_min_u32(unsigned v) {
-    v = min(v, (unsigned)VB_DPP(v, 0xB1, 0xF));   // quad_perm [1,0,3,2]
-    v = min(v, (unsigned)VB_DPP(v, 0x4E, 0xF));   // quad_perm [2,3,0,1]
-    v = min(v, (unsigned)VB_DPP(v, 0x141, 0xF));  // row_half_mirror
-    v = min(v, (unsigned)VB_DPP(v, 0x140, 0xF));  // row_mirror
-    v = min(v, (unsigned)VB_DPP(v, 0x142, 0xA));  // row_bcast15
-    v = min(v, (unsigned)VB_DPP(v, 0x143, 0xC));  // row_bcast31
+    v = min(v, (unsigned)VB_DPPMIN(v, 0xB1, 0xF));   // quad_perm [1,0,3,2]
+    v = min(v, (unsigned)VB_DPPMIN(v, 0x4E, 0xF));   // quad_perm [2,3,0,1]
+    v = min(v, (unsigned)VB_DPPMIN(v, 0x141, 0xF));  // row_half_mirror
+    v = min(v, (unsigned)VB_DPPMIN(v, 0x140, 0xF));  // row_mirror
+    v = min(v, (unsigned)VB_DPPMIN(v, 0x142, 0xA));  // row_bcast15
+    v = min(v, (unsigned)VB_DPPMIN(v, 0x143, 0xC));  // row_bcast31
     return (unsigned)__builtin_amdgcn_readlane((int)v, 63);
 }
 
