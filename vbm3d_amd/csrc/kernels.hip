@@ -1491,31 +1491,32 @@ __device__ __forceinline__ void agg_tile(const A& a, const int bx, const int by,
                         jj[u] = m ? j0 + (int)__builtin_ctzll(m) : -1;
                         m &= m - 1;
                     }
+                    // straight-line loads: an absent entry repeats the
+                    // first one, a lane outside a patch reads the patch's
+                    // first word; only the sums below are masked
                     float ev[U][C], kai[U], w[U][C];
                     bool on[U];
 #pragma unroll
                     for (int u = 0; u < U; u++) {
-                        on[u] = false;
-                        kai[u] = 0.f;
-#pragma unroll
-                        for (int c = 0; c < C; c++) ev[u][c] = w[u][c] = 0.f;
-                        if (jj[u] < 0) continue;
-                        const uint32_t pos = __builtin_amdgcn_readfirstlane(L_pos[jj[u]]);
-                        const uint32_t sn = __builtin_amdgcn_readfirstlane(L_sn[jj[u]]);
+                        const int jv = jj[u] >= 0 ? jj[u] : jj[0];
+                        const uint32_t pos = __builtin_amdgcn_readfirstlane(L_pos[jv]);
+                        const uint32_t sn = __builtin_amdgcn_readfirstlane(L_sn[jv]);
                         const uint32_t sl = sn / kMaxN, n = sn % kMaxN;
                         const cfloat* wb = (const cfloat*)(a.wgt + (size_t)sl * C);
 #pragma unroll
                         for (int c = 0; c < C; c++) w[u][c] = RAW ? 0.f : wb[c];
                         const int dx = px - (int)(pos & 0xFFFFu), dy = py - (int)(pos >> 16);
-                        on[u] = inside && (unsigned)dx < (unsigned)K && (unsigned)dy < (unsigned)K;
-                        if (on[u]) {
-                            const int kk = dy * K + dx;
-                            kai[u] = s_kai[kk];
-                            const float* eb = a.est + ((size_t)sl * C * N + n) * K2 + (unsigned)kk;
+                        on[u] = (jj[u] >= 0) & inside & ((unsigned)dx < (unsigned)K) & ((unsigned)dy < (unsigned)K);
+                        const unsigned kk = on[u] ? (unsigned)(dy * K + dx) : 0u;
+                        kai[u] = s_kai[kk];
+                        // wave-uniform base, 32-bit lane byte offset
+                        const char* eb = (const char*)(a.est + ((size_t)sl * C * N + n) * K2);
 #pragma unroll
-                            for (int c = 0; c < C; c++) ev[u][c] = eb[(size_t)c * N * K2];
-                        }
+                        for (int c = 0; c < C; c++)
+                            ev[u][c] = *(const float*)(eb + (size_t)c * N * K2 * 4 + kk * 4u);
                     }
+                    // keeps the loads above from being sunk to their sums
+                    asm volatile("" ::: "memory");
 #pragma unroll
                     for (int u = 0; u < U; u++) {
                         if (!on[u]) continue;
