@@ -196,6 +196,13 @@ int vbm3d_autotune_batch(vbm3d_ctx* ctx, const vbm3d_clip* clips,
 long vbm3d_grid(unsigned W, unsigned H, unsigned k, unsigned p, int n,
                 int nb, unsigned* xs, unsigned* ys, unsigned* crop);
 
+/* Frames of reference patches per chunk for the per-clip entries
+ * (vbm3d_run, vbm3d_run_device, vbm3d_step): 0, the default, takes as
+ * many as the memory budget allows; 1 launches every kernel once per
+ * frame, the plain per-frame path.  Results do not depend on it.  No
+ * counterpart in the reference, whose loops run per patch on the CPU. */
+int vbm3d_set_chunk_frames(vbm3d_ctx* ctx, unsigned frames);
+
 /* Kernel timing and work counters (a diagnostic with no counterpart in
  * the reference, for roofline measurements).  vbm3d_profile(ctx, 1)
  * resets the counters and brackets every later launch with HIP events on

@@ -194,6 +194,17 @@ def test_run_chunked_ring(cfg1, oracle, gpu_ctx, monkeypatch):
         _run_both(noisy, 20.0, oracle, ctx)
 
 
+@pytest.mark.parametrize("frames", [1, 3])
+def test_run_per_frame_chunks(small_rgb, oracle, frames):
+    """vbm3d_set_chunk_frames: the plain per-frame path (one launch per
+    kernel and frame) and 3-frame chunks give the oracle's bits."""
+    import vbm3d_amd as V
+    _, noisy = small_rgb
+    with V.Context(0) as ctx:
+        ctx.set_chunk_frames(frames)
+        _run_both(noisy, 20.0, oracle, ctx)
+
+
 def test_run_config2_threaded_grid(oracle, gpu_ctx):
     """30 RGB frames 960x540, sigma 20, seed 1, against the oracle run with
     the OpenMP grid of 16 threads (ref_grid_nb = 16), bit for bit."""

@@ -110,6 +110,7 @@ def lib():
             L.vbm3d_grid.restype = C.c_long
             L.vbm3d_selftest_partial_sort.argtypes = [fp, up, C.c_uint, C.c_uint]
             L.vbm3d_profile.argtypes = [C.c_void_p, C.c_int]
+            L.vbm3d_set_chunk_frames.argtypes = [C.c_void_p, C.c_uint]
             L.vbm3d_profile_read.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double),
                                              C.POINTER(C.c_long), C.POINTER(C.c_double)]
             _lib = L
@@ -154,6 +155,13 @@ class Context:
         self.h = lib().vbm3d_create(device)
         if not self.h:
             raise VBM3DError(-2, lib().vbm3d_last_error().decode())
+
+    def set_chunk_frames(self, frames: int):
+        """Frames of reference patches per chunk for the per-clip entries:
+        0 (default) = as many as the memory budget allows, 1 = one launch
+        per kernel and frame (the plain per-frame path).  Results do not
+        depend on it."""
+        _check(lib().vbm3d_set_chunk_frames(self.h, int(frames)))
 
     def close(self):
         if self.h:

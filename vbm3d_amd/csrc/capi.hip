@@ -253,6 +253,7 @@ struct vbm3d_ctx {
     DevBuf noisy, basic, den, ff, bf, num_raw, den_raw;  // host-buffer entry points
     DevBuf idx, nsx, nsim, dist, est, wgt, tabs, refxy, meta, gpos, rlist, scal, stats, counts;
     size_t est_budget = 0;  // bytes for the per-patch rings; 0 = from free device memory
+    unsigned chunk_frames = 0;  // per-clip path: frames per chunk; 0 = from the budget
     // kernel profiling (vbm3d_profile / vbm3d_profile_read)
     bool prof = false;    // bracket launches with events
     bool count = false;   // also accumulate work counters (atomics; not for timed runs)
@@ -345,6 +346,7 @@ int run_step(vbm3d_ctx* ctx, const StepIO& io, hipStream_t s) {
     const size_t budget = ring_budget(ctx, ctx->est.cap + ctx->idx.cap + ctx->nsx.cap + ctx->wgt.cap);
     size_t cap_frames = std::max<size_t>(1, budget / std::max<size_t>(per_frame, 1));
     unsigned S = (unsigned)std::max<long>(1, (long)cap_frames - 2 * (long)p.Nf);
+    if (ctx->chunk_frames) S = std::min(S, ctx->chunk_frames);
     unsigned ring = std::min<unsigned>(Fr, S + 2 * p.Nf);
     if (S >= Fr) { S = Fr; ring = Fr; }
     const size_t slots = (size_t)ring * nrf;
@@ -1216,6 +1218,12 @@ int vbm3d_block_match(vbm3d_ctx* ctx, const float* vid, unsigned W, unsigned H, 
     HIP_TRY(hipMemcpy(dist_out, ctx->dist.p, nrefs * prm->N * 4, hipMemcpyDeviceToHost));
     HIP_TRY(hipMemcpy(nsim_out, ctx->nsim.p, nrefs * 4, hipMemcpyDeviceToHost));
     HIP_TRY(hipMemcpy(nsx_out, ctx->nsx.p, nrefs * 4, hipMemcpyDeviceToHost));
+    return VBM3D_OK;
+}
+
+int vbm3d_set_chunk_frames(vbm3d_ctx* ctx, unsigned frames) {
+    if (!ctx) return fail(VBM3D_ERR_ARG, "invalid argument");
+    ctx->chunk_frames = frames;
     return VBM3D_OK;
 }
 
