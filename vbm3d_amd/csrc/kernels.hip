@@ -495,6 +495,41 @@ __device__ __forceinline__ void bm_one(const A& a, const long r, float* base, co
             wave_sync();
             for (int ch = 0; ch < 2; ch++) {
                 if (!act[ch]) continue;
+                // The reference takes the top Nb of each window, then the top
+                // Nb of those picks.  When each of the chain's Nb smallest
+                // keys is held by one slot only, that equals the Nb smallest
+                // over all of the chain's slots (the window ranks cannot
+                // matter), found with Nb wave minimums; a repeated key falls
+                // through to the exact selection below.
+                {
+                    const unsigned MX = 0xFFFFFFFFu;
+                    unsigned key = ((bal[ch] >> lane) & 1ull) ? fkey(s_d[ch * 64 + lane]) : MX;
+                    const unsigned k1 = wave_min_u32(key);
+                    const unsigned long long e1 = __ballot(key == k1);
+                    const int p1 = __ffsll((long long)e1) - 1;
+                    bool unique = __popcll(e1) == 1;
+                    int p2 = p1;
+                    if (Nb == 2) {
+                        if (lane == p1) key = MX;
+                        const unsigned k2 = wave_min_u32(key);
+                        const unsigned long long e2 = __ballot(key == k2);
+                        unique = unique && __popcll(e2) == 1;
+                        p2 = __ffsll((long long)e2) - 1;
+                    }
+                    if (unique) {
+                        n_eval += __popcll(bal[ch]);
+                        const int at = Nb * (1 + (ch == 0 ? f : nfw + f));
+                        if (lane < Nb) {
+                            const int sl = ch * 64 + (lane == 0 ? p1 : p2);
+                            const uint32_t iv = s_i[sl];
+                            pp[(ch == 0 ? 8 : 0) + lane] = iv;
+                            b_d[at + lane] = s_d[sl];
+                            b_i[at + lane] = iv;
+                        }
+                        wave_sync();
+                        continue;
+                    }
+                }
                 // top Nb of each window's list, appended in window order
                 int nfb = 0;
                 for (int m = 0; m < Nb; m++) {
