@@ -423,25 +423,66 @@ void bior_inv(T* sig, const TabT<T>& tb) {
         for (unsigned j = 0; j < 8; j++) sig[j * 8 + i] = a[i * 8 + j];
 }
 
-// Orthonormal 2D DCT-II as FFTW REDFT10 on both axes followed by
-// coef_norm (src/vbm3d.cpp:1245-1302).  Row pass (along columns q) first,
-// then the column pass; out[u*k+v], u = vertical frequency.
+// One 1D pass of REDFT10 on k values at stride sx, written at stride sy.
+// Row v of the matrix is even (v even) or odd (v odd) about the centre, so
+// each output takes the folded inputs s[q] = x[q] + x[k-1-q] or
+// d[q] = x[q] - x[k-1-q], q < k/2, plus the centre sample for odd k and
+// even v (the odd rows vanish there); the sum runs in q order.  This is
+// FFTW's REDFT10 (the reference's transform) up to rounding, with about
+// half the multiplications of the plain matrix product.
+template <class T>
+void dct_pass_fwd(const T* x, unsigned sx, T* y, unsigned sy, const TabT<T>& tb) {
+    const unsigned k = tb.k, h = k / 2;
+    T s[8], d[8];
+    for (unsigned q = 0; q < h; q++) {
+        s[q] = x[q * sx] + x[(k - 1 - q) * sx];
+        d[q] = x[q * sx] - x[(k - 1 - q) * sx];
+    }
+    for (unsigned v = 0; v < k; v++) {
+        T m = 0;
+        if (v % 2 == 0) {
+            for (unsigned q = 0; q < h; q++) m += s[q] * tb.dctF[v * k + q];
+            if (k & 1) m += x[h * sx] * tb.dctF[v * k + h];
+        } else {
+            for (unsigned q = 0; q < h; q++) m += d[q] * tb.dctF[v * k + q];
+        }
+        y[v * sy] = m;
+    }
+}
+
+// One 1D pass of REDFT01 (matrix I[o][i] = dctI[o*k + i]).  Rows o and
+// k-1-o differ only in the sign of the odd-i terms: E = sum over even i,
+// O = sum over odd i (each in i order), y[o] = E + O, y[k-1-o] = E - O;
+// the centre row of odd k is E alone (its odd terms vanish).
+template <class T>
+void dct_pass_inv(const T* z, unsigned sz, T* y, unsigned sy, const TabT<T>& tb) {
+    const unsigned k = tb.k, h = k / 2;
+    for (unsigned o = 0; o < h; o++) {
+        T e = 0, od = 0;
+        for (unsigned i = 0; i < k; i += 2) e += z[i * sz] * tb.dctI[o * k + i];
+        for (unsigned i = 1; i < k; i += 2) od += z[i * sz] * tb.dctI[o * k + i];
+        y[o * sy] = e + od;
+        y[(k - 1 - o) * sy] = e - od;
+    }
+    if (k & 1) {
+        T e = 0;
+        for (unsigned i = 0; i < k; i += 2) e += z[i * sz] * tb.dctI[h * k + i];
+        y[h * sy] = e;
+    }
+}
+
+// Orthonormal 2D DCT-II as REDFT10 on both axes followed by coef_norm
+// (src/vbm3d.cpp:1245-1302).  Row pass (along columns q) first, then the
+// column pass; out[u*k+v], u = vertical frequency.
 template <class T>
 void dct_fwd(const T* x, T* out, const TabT<T>& tb) {
     const unsigned k = tb.k;
-    T t[256];
-    for (unsigned p = 0; p < k; p++)
-        for (unsigned v = 0; v < k; v++) {
-            T m = 0;
-            for (unsigned q = 0; q < k; q++) m += x[p * k + q] * tb.dctF[v * k + q];
-            t[p * k + v] = m;
-        }
-    for (unsigned u = 0; u < k; u++)
-        for (unsigned v = 0; v < k; v++) {
-            T m = 0;
-            for (unsigned p = 0; p < k; p++) m += t[p * k + v] * tb.dctF[u * k + p];
-            out[u * k + v] = m * tb.nrm[u * k + v];
-        }
+    T t[256], c[16];
+    for (unsigned p = 0; p < k; p++) dct_pass_fwd(x + p * k, 1, t + p * k, 1, tb);
+    for (unsigned v = 0; v < k; v++) {
+        dct_pass_fwd(t + v, k, c, 1, tb);
+        for (unsigned u = 0; u < k; u++) out[u * k + v] = c[u] * tb.nrm[u * k + v];
+    }
 }
 
 // dct_2d_inv (src/vbm3d.cpp:1678-1713): coef_norm_inv, REDFT01 on both
@@ -449,20 +490,13 @@ void dct_fwd(const T* x, T* out, const TabT<T>& tb) {
 template <class T>
 void dct_inv(T* sig, const TabT<T>& tb) {
     const unsigned k = tb.k;
-    T z[256], t[256];
+    T z[256], t[256], c[16];
     for (unsigned a = 0; a < k * k; a++) z[a] = sig[a] * tb.inv[a];
-    for (unsigned u = 0; u < k; u++)
-        for (unsigned q = 0; q < k; q++) {
-            T m = 0;
-            for (unsigned v = 0; v < k; v++) m += z[u * k + v] * tb.dctI[q * k + v];
-            t[u * k + q] = m;
-        }
-    for (unsigned p = 0; p < k; p++)
-        for (unsigned q = 0; q < k; q++) {
-            T m = 0;
-            for (unsigned u = 0; u < k; u++) m += t[u * k + q] * tb.dctI[p * k + u];
-            sig[p * k + q] = tb.dct_out * m;
-        }
+    for (unsigned u = 0; u < k; u++) dct_pass_inv(z + u * k, 1, t + u * k, 1, tb);
+    for (unsigned q = 0; q < k; q++) {
+        dct_pass_inv(t + q, k, c, 1, tb);
+        for (unsigned p = 0; p < k; p++) sig[p * k + q] = tb.dct_out * c[p];
+    }
 }
 
 // ---------------------------------------------------------------------

@@ -927,6 +927,77 @@ __device__ __forceinline__ float wiener_sum(const float* vc) {
     return acc;
 }
 
+// One 1D pass of REDFT10 on K values: row v of the matrix is even (v
+// even) or odd (v odd) about the centre, so each output takes the folded
+// inputs x[q] + x[K-1-q] or x[q] - x[K-1-q], q < K/2, plus the centre
+// sample for odd K and even v; sums in q order, as the oracle's
+// dct_pass_fwd.  About half the multiplications of the matrix product.
+template <int K, class TAB>
+__device__ __forceinline__ void dct_fwd_pass(const float* x, float* y, const TAB* F) {
+    constexpr int H = K / 2;
+    float s[H], d[H];
+#pragma unroll
+    for (int q = 0; q < H; q++) {
+        s[q] = x[q] + x[K - 1 - q];
+        d[q] = x[q] - x[K - 1 - q];
+    }
+#pragma unroll
+    for (int v = 0; v < K; v++) {
+        float m = 0.f;
+        if ((v & 1) == 0) {
+#pragma unroll
+            for (int q = 0; q < H; q++) m = m + s[q] * F[v * K + q];
+            if (K & 1) m = m + x[H] * F[v * K + H];
+        } else {
+#pragma unroll
+            for (int q = 0; q < H; q++) m = m + d[q] * F[v * K + q];
+        }
+        y[v] = m;
+    }
+}
+
+// Output v of dct_fwd_pass on x[0], x[stride], ... (the same folding and
+// the same order of operations, so the two agree bit for bit).
+template <int K, class TAB>
+__device__ __forceinline__ float dct_fwd_one(const float* x, int stride, int v, const TAB* F) {
+    constexpr int H = K / 2;
+    float m = 0.f;
+    if ((v & 1) == 0) {
+#pragma unroll
+        for (int q = 0; q < H; q++) m = m + (x[q * stride] + x[(K - 1 - q) * stride]) * F[v * K + q];
+        if (K & 1) m = m + x[H * stride] * F[v * K + H];
+    } else {
+#pragma unroll
+        for (int q = 0; q < H; q++) m = m + (x[q * stride] - x[(K - 1 - q) * stride]) * F[v * K + q];
+    }
+    return m;
+}
+
+// One 1D pass of REDFT01 (I[o][i] = table[o*K + i]): rows o and K-1-o
+// differ only in the sign of the odd-i terms, so E (even i) and O (odd i)
+// give y[o] = E + O and y[K-1-o] = E - O; the centre row of odd K is E.
+// Sums in i order, as the oracle's dct_pass_inv.
+template <int K, class TAB>
+__device__ __forceinline__ void dct_inv_pass(const float* z, float* y, const TAB* I) {
+    constexpr int H = K / 2;
+#pragma unroll
+    for (int o = 0; o < H; o++) {
+        float e = 0.f, od = 0.f;
+#pragma unroll
+        for (int i = 0; i < K; i += 2) e = e + z[i] * I[o * K + i];
+#pragma unroll
+        for (int i = 1; i < K; i += 2) od = od + z[i] * I[o * K + i];
+        y[o] = e + od;
+        y[K - 1 - o] = e - od;
+    }
+    if (K & 1) {
+        float e = 0.f;
+#pragma unroll
+        for (int i = 0; i < K; i += 2) e = e + z[i] * I[H * K + i];
+        y[H] = e;
+    }
+}
+
 // Transposition buffer: patch n of a lane batch is a K x (K+1) tile at
 // n*xstride(K) (row pitch K+1).  The strides keep both the row-wise write and
 // the column-wise read of the 64 lanes (n, r) = n*K + r bank-conflict free
@@ -1024,12 +1095,19 @@ __device__ __forceinline__ void filter_one(const A& a, const long r, float* X, c
                         for (int q = 0; q < K; q++) x[q] = row[q];
                     }
                     // pass 1 along the row: t[v] = sum_q x[q] M[v][q]
+                    if (bior) {
 #pragma unroll
-                    for (int v = 0; v < K; v++) {
-                        float m = 0.f;
+                        for (int v = 0; v < K; v++) {
+                            float m = 0.f;
 #pragma unroll
-                        for (int q = 0; q < K; q++) m = m + x[q] * TF[v * K + q];
-                        X[nl * XS + v * XP + rr] = m;  // stored transposed
+                            for (int q = 0; q < K; q++) m = m + x[q] * TF[v * K + q];
+                            X[nl * XS + v * XP + rr] = m;  // stored transposed
+                        }
+                    } else {
+                        float y[K];
+                        dct_fwd_pass<K>(x, y, TF);
+#pragma unroll
+                        for (int v = 0; v < K; v++) X[nl * XS + v * XP + rr] = y[v];
                     }
                 }
                 wave_sync();
@@ -1038,15 +1116,21 @@ __device__ __forceinline__ void filter_one(const A& a, const long r, float* X, c
                     float col[K];
 #pragma unroll
                     for (int p = 0; p < K; p++) col[p] = X[nl * XS + rr * XP + p];
+                    // BIOR keeps out[i*8+j] = (column i, output j); the DCT
+                    // writes out[u*K+v] scaled by coef_norm
+                    if (bior) {
 #pragma unroll
-                    for (int u = 0; u < K; u++) {
-                        float m = 0.f;
+                        for (int u = 0; u < K; u++) {
+                            float m = 0.f;
 #pragma unroll
-                        for (int p = 0; p < K; p++) m = m + col[p] * TF[u * K + p];
-                        // BIOR keeps out[i*8+j] = (column i, output j); the
-                        // DCT writes out[u*K+v] scaled by coef_norm
-                        if (bior) dst[n * K2 + rr * 8 + u] = m;
-                        else dst[n * K2 + u * K + rr] = m * NRM[u * K + rr];
+                            for (int p = 0; p < K; p++) m = m + col[p] * TF[u * K + p];
+                            dst[n * K2 + rr * 8 + u] = m;
+                        }
+                    } else {
+                        float y[K];
+                        dct_fwd_pass<K>(col, y, TF);
+#pragma unroll
+                        for (int u = 0; u < K; u++) dst[n * K2 + u * K + rr] = y[u] * NRM[u * K + rr];
                     }
                 }
                 wave_sync();
@@ -1077,12 +1161,19 @@ __device__ __forceinline__ void filter_one(const A& a, const long r, float* X, c
 #pragma unroll
                 for (int s2 = 0; s2 < K; s2++)
                     x[s2] = bior ? est[n * K2 + s2 * 8 + rr] : est[n * K2 + rr * K + s2] * INV[rr * K + s2];
+                if (bior) {
 #pragma unroll
-                for (int v = 0; v < K; v++) {
-                    float m = 0.f;
+                    for (int v = 0; v < K; v++) {
+                        float m = 0.f;
 #pragma unroll
-                    for (int q = 0; q < K; q++) m = m + x[q] * TI[v * K + q];
-                    X[nl * XS + v * XP + rr] = m;
+                        for (int q = 0; q < K; q++) m = m + x[q] * TI[v * K + q];
+                        X[nl * XS + v * XP + rr] = m;
+                    }
+                } else {
+                    float y[K];
+                    dct_inv_pass<K>(x, y, TI);
+#pragma unroll
+                    for (int v = 0; v < K; v++) X[nl * XS + v * XP + rr] = y[v];
                 }
             }
             wave_sync();
@@ -1090,14 +1181,21 @@ __device__ __forceinline__ void filter_one(const A& a, const long r, float* X, c
                 float col[K];
 #pragma unroll
                 for (int p = 0; p < K; p++) col[p] = X[nl * XS + rr * XP + p];
+                // BIOR: pixel (row u, column rr); DCT: pixel (row u, column
+                // rr) after the 1/(2k) factor
+                if (bior) {
 #pragma unroll
-                for (int u = 0; u < K; u++) {
-                    float m = 0.f;
+                    for (int u = 0; u < K; u++) {
+                        float m = 0.f;
 #pragma unroll
-                    for (int p = 0; p < K; p++) m = m + col[p] * TI[u * K + p];
-                    // BIOR: pixel (row u, column rr); DCT: pixel (row u,
-                    // column rr) after the 1/(2k) factor
-                    dstg[(size_t)n * K2 + u * K + rr] = bior ? m : dct_out * m;
+                        for (int p = 0; p < K; p++) m = m + col[p] * TI[u * K + p];
+                        dstg[(size_t)n * K2 + u * K + rr] = m;
+                    }
+                } else {
+                    float y[K];
+                    dct_inv_pass<K>(col, y, TI);
+#pragma unroll
+                    for (int u = 0; u < K; u++) dstg[(size_t)n * K2 + u * K + rr] = dct_out * y[u];
                 }
             }
             wave_sync();
@@ -1206,16 +1304,11 @@ __device__ float adj_fwd2d(const float* in, float* tmp, int lane, bool bior, con
         if (act)
             for (int s = 0; s < 8; s++) o = o + tmp[s * 8 + a] * T.biorF[b * 8 + s];
     } else {
-        if (act) {
-            float m = 0.f;
-            for (int q = 0; q < K; q++) m = m + in[a * K + q] * T.dctF[b * K + q];
-            tmp[lane] = m;
-        }
+        // the forward filter's folded passes, so that the Wiener gains
+        // taken from these coefficients are the forward pass's bits
+        if (act) tmp[lane] = dct_fwd_one<K>(in + a * K, 1, b, T.dctF);
         wave_sync();
-        if (act) {
-            for (int p = 0; p < K; p++) o = o + tmp[p * K + b] * T.dctF[a * K + p];
-            o = o * T.nrm[lane];
-        }
+        if (act) o = dct_fwd_one<K>(tmp + b, K, a, T.dctF) * T.nrm[lane];
     }
     wave_sync();
     return o;
