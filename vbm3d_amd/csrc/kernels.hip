@@ -1565,10 +1565,18 @@ __device__ __forceinline__ void agg_tile(const A& a, const int bx, const int by,
                 const unsigned tq = div32(rest, dnr), ii = rest - tq * nr;
                 const unsigned tr = trlo + tq;
                 slot = (tr % a.ring) * a.nrf + off + (ia + ii) * ncols + (ja + jj);
+                // the member indices are loaded with the group size, not
+                // after it (one memory round trip); members n >= nSx are
+                // masked afterwards
                 const int ns = (int)a.nsx[slot];
                 const uint32_t* mem = a.idx + (size_t)slot * N;
-                for (int n = 0; n < ns; n++) {
-                    const unsigned r = mem[n] - tbase;
+                uint32_t mi[kMaxN];
+#pragma unroll
+                for (int n = 0; n < kMaxN; n++) mi[n] = n < N ? mem[n] : 0u;
+#pragma unroll
+                for (int n = 0; n < kMaxN; n++) {
+                    if (n >= ns) break;
+                    const unsigned r = mi[n] - tbase;
                     if (r < rmin || r >= rmax) continue;
                     const int qy = (int)div32(r, dW), qx = (int)(r - (unsigned)qy * W);
                     if (qx <= x0 + TS - 1 && qx + K - 1 >= x0) hits |= 1u << n;
