@@ -1510,7 +1510,7 @@ __device__ __forceinline__ int lower_bound_u16(const uint16_t* a, int n, int v) 
 // accumulates, for its 8x8 quadrant, the members that overlap the quadrant
 // in list order: every output sample sees its contributions in exactly the
 // reference's order.
-template <int C, bool RAW, class A>
+template <int C, bool RAW, int NM, class A>
 __device__ __forceinline__ void agg_tile(const A& a, const int bx, const int by, const unsigned bz, const int nl) {
     typedef const __attribute__((address_space(4))) float cfloat;
     constexpr int TS = kAggTile;
@@ -1557,30 +1557,43 @@ __device__ __forceinline__ void agg_tile(const A& a, const int bx, const int by,
 #pragma unroll
         for (int c = 0; c < C; c++) pn[c] = pd[c] = 0.f;
 
-        for (long cb = 0; cb < total; cb += 256) {
+        // The scan data of a chunk (slot, group size, member indices, up
+        // to NM >= N of them) is loaded one chunk ahead: the loads for chunk
+        // cb + 256 fly while chunk cb is accumulated.
+        uint32_t slot_n = 0, mi_n[NM];
+        int ns_n = 0;
+        auto fetch = [&](long cb) {
             const unsigned e = (unsigned)(cb + tid);
-            uint32_t hits = 0, slot = 0;
+            slot_n = 0;
+            ns_n = 0;
+#pragma unroll
+            for (int n = 0; n < NM; n++) mi_n[n] = 0u;
             if (e < total) {
                 const unsigned rest = div32(e, dnc), jj = e - rest * nc;
                 const unsigned tq = div32(rest, dnr), ii = rest - tq * nr;
                 const unsigned tr = trlo + tq;
-                slot = (tr % a.ring) * a.nrf + off + (ia + ii) * ncols + (ja + jj);
-                // the member indices are loaded with the group size, not
-                // after it (one memory round trip); members n >= nSx are
-                // masked afterwards
-                const int ns = (int)a.nsx[slot];
-                const uint32_t* mem = a.idx + (size_t)slot * N;
-                uint32_t mi[kMaxN];
+                slot_n = (tr % a.ring) * a.nrf + off + (ia + ii) * ncols + (ja + jj);
+                ns_n = (int)a.nsx[slot_n];
+                const uint32_t* mem = a.idx + (size_t)slot_n * N;
 #pragma unroll
-                for (int n = 0; n < kMaxN; n++) mi[n] = n < N ? mem[n] : 0u;
+                for (int n = 0; n < NM; n++) mi_n[n] = n < N ? mem[n] : 0u;
+            }
+        };
+        if (total > 0) fetch(0);
+        for (long cb = 0; cb < total; cb += 256) {
+            const uint32_t slot = slot_n;
+            const int ns = ns_n;
+            uint32_t mi[NM];
 #pragma unroll
-                for (int n = 0; n < kMaxN; n++) {
-                    if (n >= ns) break;
-                    const unsigned r = mi[n] - tbase;
-                    if (r < rmin || r >= rmax) continue;
-                    const int qy = (int)div32(r, dW), qx = (int)(r - (unsigned)qy * W);
-                    if (qx <= x0 + TS - 1 && qx + K - 1 >= x0) hits |= 1u << n;
-                }
+            for (int n = 0; n < NM; n++) mi[n] = mi_n[n];
+            uint32_t hits = 0;
+#pragma unroll
+            for (int n = 0; n < NM; n++) {
+                if (n >= ns) break;
+                const unsigned r = mi[n] - tbase;
+                if (r < rmin || r >= rmax) continue;
+                const int qy = (int)div32(r, dW), qx = (int)(r - (unsigned)qy * W);
+                if (qx <= x0 + TS - 1 && qx + K - 1 >= x0) hits |= 1u << n;
             }
             // ordered compaction: exclusive block scan of the hit counts
             const int cnt = __popc(hits);
@@ -1596,15 +1609,17 @@ __device__ __forceinline__ void agg_tile(const A& a, const int bx, const int by,
             for (int q = 0; q < wv; q++) base += s_w[q];
             const int tot = s_w[0] + s_w[1] + s_w[2] + s_w[3];
             int o = base + x - cnt;
-            for (int n = 0; n < N; n++)
+#pragma unroll
+            for (int n = 0; n < NM; n++)
                 if ((hits >> n) & 1u) {
-                    const unsigned r = a.idx[(size_t)slot * N + n] - tbase;
+                    const unsigned r = mi[n] - tbase;
                     const unsigned qy = div32(r, dW), qx = r - qy * W;
                     L_sn[o] = slot * kMaxN + n;
                     L_pos[o] = qx | (qy << 16);
                     o++;
                 }
             __syncthreads();
+            if (cb + 256 < total) fetch(cb + 256);
             // this wave: the list entries overlapping its quadrant, in order
             for (int j0 = 0; j0 < tot; j0 += 64) {
                 bool ov = false;
@@ -1693,14 +1708,14 @@ __device__ __forceinline__ void agg_tile(const A& a, const int bx, const int by,
     }
 }
 
-template <int C, bool RAW>
+template <int C, bool RAW, int NM>
 __global__ __launch_bounds__(256) void k_aggregate(AArgs a) {
-    agg_tile<C, RAW>(a, blockIdx.x, blockIdx.y, blockIdx.z, a.N);
+    agg_tile<C, RAW, NM>(a, blockIdx.x, blockIdx.y, blockIdx.z, a.N);
 }
 
 // Persistent form: one tile per take (the workgroup is the unit); item ->
 // (job, frame, tile row, tile column).
-template <int C>
+template <int C, int NM>
 __global__ __launch_bounds__(256) void k_aggregate_queue(const AArgs* __restrict__ jobs, WorkQueue q, int nl) {
     __shared__ long s_item;
     for (;;) {
@@ -1715,7 +1730,7 @@ __global__ __launch_bounds__(256) void k_aggregate_queue(const AArgs* __restrict
         const long local = it - q.item_begin[j];
         const long tx = (a.W + kAggTile - 1) / kAggTile, ty = (a.H + kAggTile - 1) / kAggTile;
         const long bz = local / (tx * ty), rem = local - bz * tx * ty;
-        agg_tile<C, false>(a, (int)(rem % tx), (int)(rem / tx), (unsigned)bz, nl);
+        agg_tile<C, false, NM>(a, (int)(rem % tx), (int)(rem / tx), (unsigned)bz, nl);
     }
 }
 
@@ -1995,8 +2010,14 @@ hipError_t launch_aggregate_queue(int C, const AArgs* jobs, const WorkQueue& q, 
     if (q.total <= 0 || blocks <= 0) return hipSuccess;
     if (maxN < 1 || maxN > kMaxN) return hipErrorInvalidValue;
     const size_t lds = (size_t)2 * 256 * maxN * sizeof(uint32_t);
-    if (C == 1) hipLaunchKernelGGL(k_aggregate_queue<1>, dim3(blocks), dim3(256), lds, s, jobs, q, maxN);
-    else hipLaunchKernelGGL(k_aggregate_queue<3>, dim3(blocks), dim3(256), lds, s, jobs, q, maxN);
+    // members held in registers by the scan: 8, or 16 when some job has N = 16
+    if (maxN <= 8) {
+        if (C == 1) hipLaunchKernelGGL((k_aggregate_queue<1, 8>), dim3(blocks), dim3(256), lds, s, jobs, q, maxN);
+        else hipLaunchKernelGGL((k_aggregate_queue<3, 8>), dim3(blocks), dim3(256), lds, s, jobs, q, maxN);
+    } else {
+        if (C == 1) hipLaunchKernelGGL((k_aggregate_queue<1, 16>), dim3(blocks), dim3(256), lds, s, jobs, q, maxN);
+        else hipLaunchKernelGGL((k_aggregate_queue<3, 16>), dim3(blocks), dim3(256), lds, s, jobs, q, maxN);
+    }
     return hipGetLastError();
 }
 
@@ -2014,13 +2035,21 @@ hipError_t launch_aggregate(int C, const AArgs& a, unsigned frames, hipStream_t 
     dim3 grid((a.W + kAggTile - 1) / kAggTile, (a.H + kAggTile - 1) / kAggTile, frames);
     if (a.N < 1 || a.N > kMaxN) return hipErrorInvalidValue;
     const size_t lds = (size_t)2 * 256 * a.N * sizeof(uint32_t);  // the chunk's ordered list
-    if (a.raw) {
-        if (C == 1) hipLaunchKernelGGL((k_aggregate<1, true>), grid, dim3(256), lds, s, a);
-        else hipLaunchKernelGGL((k_aggregate<3, true>), grid, dim3(256), lds, s, a);
-    } else {
-        if (C == 1) hipLaunchKernelGGL((k_aggregate<1, false>), grid, dim3(256), lds, s, a);
-        else hipLaunchKernelGGL((k_aggregate<3, false>), grid, dim3(256), lds, s, a);
+    // members held in registers by the scan: 8, or 16 for N = 16
+#define VB_AGG(NM)                                                                               \
+    if (a.raw) {                                                                                 \
+        if (C == 1) hipLaunchKernelGGL((k_aggregate<1, true, NM>), grid, dim3(256), lds, s, a);  \
+        else hipLaunchKernelGGL((k_aggregate<3, true, NM>), grid, dim3(256), lds, s, a);         \
+    } else {                                                                                     \
+        if (C == 1) hipLaunchKernelGGL((k_aggregate<1, false, NM>), grid, dim3(256), lds, s, a); \
+        else hipLaunchKernelGGL((k_aggregate<3, false, NM>), grid, dim3(256), lds, s, a);        \
     }
+    if (a.N <= 8) {
+        VB_AGG(8)
+    } else {
+        VB_AGG(16)
+    }
+#undef VB_AGG
     return hipGetLastError();
 }
 
