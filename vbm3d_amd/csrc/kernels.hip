@@ -326,7 +326,26 @@ __device__ __forceinline__ void bm_one(const A& a, const long r, float* base, co
         const int x1 = min((int)W - K, (int)rx + h), y1 = min((int)H - K, (int)ry + h);
         const int wx = x1 - x0 + 1, wy = y1 - y0 + 1, m = wx * wy;
         const int cs = L.tp0 * L.th0;
-        load_tile(tile, L.tp0, cs, x0, y0, wx, wy, rt);
+        {
+            // the (wx+K-1) x (wy+K-1) area, memory to LDS: word e = k*64 +
+            // lane of a channel's area is its row e / tp0, column e % tp0
+            const int aw = wx + K - 1, ah = wy + K - 1;
+            const unsigned fb = rt * whc + (unsigned)y0 * W + (unsigned)x0;
+            const unsigned tl0 = lds_addr(tile);
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                if (k * 64 >= cs) break;
+                const int e = k * 64 + lane;
+                const int er = small_div(e, L.tp0), ec = e - er * L.tp0;
+                if (er < ah && ec < aw) {
+                    const unsigned go = ((unsigned)er * W + (unsigned)ec) * 4u;
+#pragma unroll
+                    for (int c = 0; c < C; c++)
+                        lds_dma((const char*)(vid + fb + c * wh) + go, tl0 + (unsigned)((c * cs + k * 64) * 4));
+                }
+            }
+            __builtin_amdgcn_s_waitcnt(0);
+        }
         wave_sync();
         if (lane < m) {
             const int lx = lane % wx, ly = lane / wx;
@@ -456,28 +475,25 @@ __device__ __forceinline__ void bm_one(const A& a, const long r, float* base, co
             unsigned long long bal[2];
 #pragma unroll
             for (int h = 0; h < 2; h++) {
-                const int m = sm;
-                bool valid = false;
-                if (act[h] && m < Nb && sl < P2) {
-                    const int wn = h * Nb + m;
-                    const int x0 = win[wn * 5], y0 = win[wn * 5 + 1], wx = win[wn * 5 + 2], wy = win[wn * 5 + 3];
-                    if (slx < wx && sly < wy) {
-                        const int ly = sly, lx = slx;
-                        const int qx = x0 + lx, qy = y0 + ly;
-                        // already seen = inside an earlier window of this frame
-                        bool seen = false;
-                        for (int m2 = 0; m2 < m; m2++) {
-                            const int* w2 = win + (h * Nb + m2) * 5;
-                            seen |= qx >= w2[0] && qx < w2[0] + w2[2] && qy >= w2[1] && qy < w2[1] + w2[3];
-                        }
-                        valid = !seen;
-                        if (valid) {
-                            const unsigned toff = (unsigned)(wn * C * cs1 + ly * L.tp1 + lx);
-                            const bool self = (unsigned)qx == rx && (unsigned)qy == ry;
-                            s_d[h * 64 + lane] = __uint_as_float(toff | (self ? 0x80000000u : 0u));
-                            s_i[h * 64 + lane] = (unsigned)win[wn * 5 + 4] * whc + qy * W + qx;
-                        }
-                    }
+                // the chain's windows 0 and 1 (window 0 again when Nb = 1),
+                // read at wave-uniform addresses; lanes of window 1 select
+                // its values, and a candidate inside window 0 was seen
+                const int* wa = win + (h * Nb) * 5;
+                const int* wb = win + (h * Nb + (Nb > 1 ? 1 : 0)) * 5;
+                const int ax0 = wa[0], ay0 = wa[1], awx = wa[2], awy = wa[3];
+                const int bx0 = wb[0], by0 = wb[1], bwx = wb[2], bwy = wb[3];
+                const bool m1 = sm == 1;
+                const int x0 = m1 ? bx0 : ax0, y0 = m1 ? by0 : ay0;
+                const int wx = m1 ? bwx : awx, wy = m1 ? bwy : awy;
+                const int qx = x0 + slx, qy = y0 + sly;
+                const bool seen = m1 & (qx >= ax0) & (qx < ax0 + awx) & (qy >= ay0) & (qy < ay0 + awy);
+                const bool valid = act[h] & (sm < Nb) & (sl < P2) & (slx < wx) & (sly < wy) & !seen;
+                if (valid) {
+                    const int wn = h * Nb + sm;
+                    const unsigned toff = (unsigned)(wn * C * cs1 + sly * L.tp1 + slx);
+                    const bool self = (unsigned)qx == rx && (unsigned)qy == ry;
+                    s_d[h * 64 + lane] = __uint_as_float(toff | (self ? 0x80000000u : 0u));
+                    s_i[h * 64 + lane] = (unsigned)(m1 ? wb[4] : wa[4]) * whc + qy * W + qx;
                 }
                 bal[h] = __ballot(valid);
                 if (valid) lst[(h ? __popcll(bal[0]) : 0) + lanes_below(bal[h], lane)] = (unsigned)(h * 64 + lane);
