@@ -1724,8 +1724,11 @@ __device__ __forceinline__ void agg_tile(const A& a, const int bx, const int by,
     }
 }
 
+// Occupancy floor of 8 waves per SIMD (64 registers, two of them spilled):
+// the kernel waits on memory more than it computes, and measured faster
+// than at its unconstrained 7 waves.
 template <int C, bool RAW, int NM>
-__global__ __launch_bounds__(256) void k_aggregate(AArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k_aggregate(AArgs a) {
     agg_tile<C, RAW, NM>(a, blockIdx.x, blockIdx.y, blockIdx.z, a.N);
 }
 
