@@ -84,6 +84,32 @@ def test_bm_rgb_colour(small_rgb, oracle, gpu_ctx):
         _bm_compare(yuv, prm, refs, oracle, gpu_ctx)
 
 
+# search shapes other than the default (Ns 7, Npr 5, Nb 2), which runs a
+# kernel compiled with those sizes fixed: the paired form with run-time
+# sizes (Nb 1, Npr 3), and the general form (Nb > 2, or 7x7 windows)
+SHAPES = [dict(Ns=7, Npr=5, Nb=1, Nf=4), dict(Ns=5, Npr=3, Nb=2, Nf=2), dict(Ns=3, Npr=3, Nb=2, Nf=1),
+          dict(Ns=7, Npr=7, Nb=2, Nf=3), dict(Ns=7, Npr=5, Nb=3, Nf=3), dict(Ns=8, Npr=8, Nb=4, Nf=2)]
+
+
+@pytest.mark.parametrize("shape", SHAPES, ids=lambda d: "ns{Ns}-npr{Npr}-nb{Nb}-nf{Nf}".format(**d))
+def test_bm_search_shapes(oracle, gpu_ctx, shape):
+    import vbm3d_amd as V
+    _, noisy = synth.noisy_clip(64, 48, 7, 3, 20.0, 21)
+    yuv = oracle.color(noisy, True)
+    for prm in (V.initialize_parameters_1(20.0, **shape), V.initialize_parameters_2(20.0, **shape)):
+        refs = all_refs(64, 48, 7, 3, prm.k, prm.p)
+        _bm_compare(yuv, prm, refs, oracle, gpu_ctx)
+
+
+def test_run_search_shape(small_rgb, oracle, gpu_ctx):
+    """Both steps end to end with a non-default search shape."""
+    import vbm3d_amd as V
+    _, noisy = small_rgb
+    shape = dict(Ns=5, Npr=3, Nb=1, Nf=2)
+    _run_both(noisy, 20.0, oracle, gpu_ctx, p1=V.initialize_parameters_1(20.0, **shape),
+              p2=V.initialize_parameters_2(20.0, **shape))
+
+
 def test_bm_ties(oracle, gpu_ctx):
     """Integer-valued clip with 3 levels: distances tie everywhere, so the
     heap-select tie order of std::partial_sort decides."""
