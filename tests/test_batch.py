@@ -111,3 +111,27 @@ def test_autotune_leaves_inputs_and_matches(env, reference):
     assert 0 <= best["waves_per_group"] <= 4 and best["grab"] >= 1 and best["groups_per_cu"] >= 1
     V.run_batch(items)  # the tuned configuration, with a graph
     _same(items, reference, "autotuned")
+
+
+@pytest.mark.parametrize("persistent", [0, 1])
+def test_batch_mixed_search_shapes(env, persistent):
+    """Clips of one kernel class with the default search shape and with
+    another share a launch, which then runs the run-time sized kernel for
+    all of them; each clip still equals its own per-clip run (where the
+    default shape runs the kernel compiled with those sizes)."""
+    torch, V, dev, clips = env
+    shape = dict(Ns=5, Npr=3, Nb=1, Nf=2)
+    picks = [1, 5]  # two RGB clips at sigma 20 (one kernel class per step); the first keeps the default shape
+    items, ref = [], []
+    for j, i in enumerate(picks):
+        W, H, F, C, s, seed = SPECS[i]
+        p1 = V.initialize_parameters_1(s, **(shape if j else {}))
+        p2 = V.initialize_parameters_2(s, **(shape if j else {}))
+        z = torch.from_numpy(clips[i]).to(dev)
+        b, f = torch.empty_like(z), torch.empty_like(z)
+        V.run_device(s, z, b, f, p1, p2, True, 1)
+        ref.append((z.cpu().numpy(), b.cpu().numpy(), f.cpu().numpy()))
+        items.append(V.BatchItem(s, torch.from_numpy(clips[i]).to(dev), p1, p2))
+    V.run_batch(items, 1, dict(chunk_frames=0, waves_per_group=0, grab=2, groups_per_cu=4, use_graph=0,
+                                persistent=persistent))
+    _same(items, ref, f"mixed shapes, persistent={persistent}")
