@@ -32,6 +32,18 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Workgroups of a launch are dealt to the 8 XCDs in turn (workgroup b runs
+// on XCD b % 8), and every XCD has its own L2.  xcd_order maps b to a
+// logical index such that XCD x takes one contiguous range of the logical
+// order, where neighbouring patches read the same data.  It is a bijection
+// on [0, n).  Measured: it speeds up the group filter; block matching and
+// aggregation run faster in the plain order, where all XCDs work on
+// neighbouring data at the same time and share it in the last-level cache.
+__device__ __forceinline__ unsigned xcd_order(unsigned b, unsigned n) {
+    const unsigned q = n >> 3, r = n & 7, x = b & 7, j = b >> 3;
+    return x * q + min(x, r) + j;
+}
+
 // A global work queue: items [0, total) of a batched launch, taken `grab`
 // at a time through an atomic counter (zeroed before the launch); job j
 // owns items [item_begin[j], item_begin[j+1]).
@@ -1248,7 +1260,7 @@ __global__ __launch_bounds__(256) void k_filter(FArgs a) {
     const int lane = threadIdx.x & 63;
     const int w = threadIdx.x >> 6;
     const int nw = blockDim.x >> 6;
-    const long r = a.r_begin + (long)blockIdx.x * nw + w;
+    const long r = a.r_begin + (long)xcd_order(blockIdx.x, gridDim.x) * nw + w;
     if (r >= a.r_end) return;
     filter_one<K, C, STEP>(a, r, smem + (size_t)w * filter_lds_words(K, C, a.N, STEP), lane);
 }
