@@ -817,7 +817,7 @@ int plan_batch(vbm3d_ctx* ctx, const vbm3d_clip* clips, int n, int nb, const vbm
                         ag.num_out = nullptr;
                         ag.den_out = nullptr;
                         ag.raw = 0;
-                        const long tiles = (long)((c.W + kAggTile - 1) / kAggTile) * ((c.H + kAggTile - 1) / kAggTile) *
+                        const long tiles = (long)((c.W + kAggTileX - 1) / kAggTileX) * ((c.H + kAggTileY - 1) / kAggTileY) *
                                            (ready - m.t_done);
                         if (c.C == 1) { ag1.push_back(ag); pre1.push_back(pre1.back() + tiles); }
                         else { ag3.push_back(ag); pre3.push_back(pre3.back() + tiles); }

@@ -10,7 +10,9 @@ constexpr int kMaxN = 16;       // largest group size the kernels hold in regist
 constexpr int kMaxWin = 8;      // largest Ns / Npr (window <= 64 candidates)
 constexpr int kMaxBest = 64;    // Nb * (2 Nf + 1) bound
 constexpr int kMaxSlots = 512;  // Nb * Npr^2 bound
-constexpr int kAggTile = 16;    // aggregation tile edge (pixels)
+constexpr int kAggTileX = 32;   // aggregation tile width (pixels)
+constexpr int kAggTileY = 16;   // aggregation tile height
+constexpr int kAggThreads = kAggTileX * kAggTileY;  // one thread per output pixel
 
 // Exact unsigned 32-bit division by a run-time constant d >= 1 with a
 // multiply-high and two shifts (Granlund and Montgomery, "Division by

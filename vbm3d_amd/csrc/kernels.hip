@@ -14,7 +14,7 @@
 //                fallback in one lane when distances tie);
 //   k_filter     one wavefront per reference patch; lane = coefficient
 //                position, the group dimension lives in registers;
-//   k_aggregate  one 16x16 output tile per workgroup; contributions are
+//   k_aggregate  one 32x16 output tile per workgroup; contributions are
 //                gathered in the reference's (crop, frame, row, column, n)
 //                order, so every pixel's sum is formed in the same order as
 //                the CPU and no atomics are used.
@@ -1531,45 +1531,47 @@ __device__ __forceinline__ int lower_bound_u16(const uint16_t* a, int n, int v) 
     return lo;
 }
 
-// One 16x16 output tile of one frame per workgroup.  The reference patches
-// whose groups can reach the tile are scanned in processing order (crop,
-// frame, row, column), 256 at a time; the group members that overlap the
-// tile are compacted into an ordered list, and each wavefront then
-// accumulates, for its 8x8 quadrant, the members that overlap the quadrant
-// in list order: every output sample sees its contributions in exactly the
-// reference's order.
+// One 32x16 output tile of one frame per workgroup (eight wavefronts).
+// The reference patches whose groups can reach the tile are scanned in
+// processing order (crop, frame, row, column), one per thread at a time;
+// the group members that overlap the tile are compacted into an ordered
+// list, and each wavefront then accumulates, for its 8x8 block, the
+// members that overlap the block in list order: every output sample sees
+// its contributions in exactly the reference's order.
 template <int C, bool RAW, int NM, class A>
 __device__ __forceinline__ void agg_tile(const A& a, const int bx, const int by, const unsigned bz, const int nl) {
     typedef const __attribute__((address_space(4))) float cfloat;
-    constexpr int TS = kAggTile;
-    constexpr int QS = TS / 2;
-    // the ordered list of one 256-patch chunk: at most nl >= N members each
+    constexpr int TX = kAggTileX, TY = kAggTileY, NT = kAggThreads;
+    constexpr int QS = 8;             // a wavefront's block edge
+    constexpr int NWV = NT / 64, WX = TX / QS;
+    static_assert(TX % QS == 0 && TY % QS == 0 && NT % 64 == 0, "tile of 8x8 blocks");
+    // the ordered list of one NT-patch chunk: at most nl >= N members each
     extern __shared__ uint32_t agg_lists[];
     uint32_t* L_sn = agg_lists;
-    uint32_t* L_pos = agg_lists + 256 * nl;
+    uint32_t* L_pos = agg_lists + NT * nl;
     __shared__ float s_kai[kMaxWin * kMaxWin * 4];
-    __shared__ int s_w[4];
+    __shared__ int s_w[NWV];
     const int tid = threadIdx.x;
     const int lane = tid & 63, wv = tid >> 6;
-    const int x0 = bx * TS, y0 = by * TS;
+    const int x0 = bx * TX, y0 = by * TY;
     const unsigned to = a.t_begin + bz;
-    const int qx0 = x0 + (wv & 1) * QS, qy0 = y0 + (wv >> 1) * QS;  // this wave's quadrant
+    const int qx0 = x0 + (wv % WX) * QS, qy0 = y0 + (wv / WX) * QS;  // this wave's block
     const int px = qx0 + (lane & (QS - 1)), py = qy0 + lane / QS;
     const bool inside = px < (int)a.W && py < (int)a.H;
     const int K = a.K, K2 = K * K, N = a.N;
     const unsigned W = a.W, wh = a.wh, whc = a.whc;
-    for (int i = tid; i < K2; i += 256) s_kai[i] = a.tabs->kaiser[i];
+    for (int i = tid; i < K2; i += NT) s_kai[i] = a.tabs->kaiser[i];
     float tn[C], td[C];
 #pragma unroll
     for (int c = 0; c < C; c++) tn[c] = td[c] = 0.f;
     const int trlo = max(0, (int)to - a.Nf), trhi = min((int)a.Fr - 1, (int)to + a.Nf);
-    const int rlo = y0 - a.R - (K - 1), rhi = y0 + TS - 1 + a.R;
-    const int clo = x0 - a.R - (K - 1), chi = x0 + TS - 1 + a.R;
+    const int rlo = y0 - a.R - (K - 1), rhi = y0 + TY - 1 + a.R;
+    const int clo = x0 - a.R - (K - 1), chi = x0 + TX - 1 + a.R;
     // a member at q = to*whc + y*W + x overlaps the tile's rows iff
     // q - to*whc lies in [ylo*W, yhi*W)
     const unsigned tbase = to * whc;
     const Div32 dW = {a.divW.m, a.divW.s1, a.divW.s2};
-    const unsigned rmin = (unsigned)max(0, y0 - K + 1) * W, rmax = (unsigned)min((int)a.H, y0 + TS) * W;
+    const unsigned rmin = (unsigned)max(0, y0 - K + 1) * W, rmax = (unsigned)min((int)a.H, y0 + TY) * W;
 
     for (int ci = 0; ci < a.ncrop; ci++) {
         const int* meta = a.crop_meta + ci * 5;
@@ -1587,7 +1589,7 @@ __device__ __forceinline__ void agg_tile(const A& a, const int bx, const int by,
 
         // The scan data of a chunk (slot, group size, member indices, up
         // to NM >= N of them) is loaded one chunk ahead: the loads for chunk
-        // cb + 256 fly while chunk cb is accumulated.
+        // cb + NT fly while chunk cb is accumulated.
         uint32_t slot_n = 0, mi_n[NM];
         int ns_n = 0;
         auto fetch = [&](long cb) {
@@ -1608,7 +1610,7 @@ __device__ __forceinline__ void agg_tile(const A& a, const int bx, const int by,
             }
         };
         if (total > 0) fetch(0);
-        for (long cb = 0; cb < total; cb += 256) {
+        for (long cb = 0; cb < total; cb += NT) {
             const uint32_t slot = slot_n;
             const int ns = ns_n;
             uint32_t mi[NM];
@@ -1621,7 +1623,7 @@ __device__ __forceinline__ void agg_tile(const A& a, const int bx, const int by,
                 const unsigned r = mi[n] - tbase;
                 if (r < rmin || r >= rmax) continue;
                 const int qy = (int)div32(r, dW), qx = (int)(r - (unsigned)qy * W);
-                if (qx <= x0 + TS - 1 && qx + K - 1 >= x0) hits |= 1u << n;
+                if (qx <= x0 + TX - 1 && qx + K - 1 >= x0) hits |= 1u << n;
             }
             // ordered compaction: exclusive block scan of the hit counts
             const int cnt = __popc(hits);
@@ -1633,9 +1635,13 @@ __device__ __forceinline__ void agg_tile(const A& a, const int bx, const int by,
             }
             if (lane == 63) s_w[wv] = x;
             __syncthreads();
-            int base = 0;
-            for (int q = 0; q < wv; q++) base += s_w[q];
-            const int tot = s_w[0] + s_w[1] + s_w[2] + s_w[3];
+            int base = 0, tot = 0;
+#pragma unroll 1
+            for (int q = 0; q < NWV; q++) {
+                const int sq = s_w[q];
+                base += q < wv ? sq : 0;
+                tot += sq;
+            }
             int o = base + x - cnt;
 #pragma unroll
             for (int n = 0; n < NM; n++)
@@ -1647,8 +1653,8 @@ __device__ __forceinline__ void agg_tile(const A& a, const int bx, const int by,
                     o++;
                 }
             __syncthreads();
-            if (cb + 256 < total) fetch(cb + 256);
-            // this wave: the list entries overlapping its quadrant, in order
+            if (cb + NT < total) fetch(cb + NT);
+            // this wave: the list entries overlapping its 8x8 block, in order
             for (int j0 = 0; j0 < tot; j0 += 64) {
                 bool ov = false;
                 if (j0 + lane < tot) {
@@ -1740,14 +1746,14 @@ __device__ __forceinline__ void agg_tile(const A& a, const int bx, const int by,
 // the kernel waits on memory more than it computes, and measured faster
 // than at its unconstrained 7 waves.
 template <int C, bool RAW, int NM>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k_aggregate(AArgs a) {
+__global__ __launch_bounds__(kAggThreads) __attribute__((amdgpu_waves_per_eu(8))) void k_aggregate(AArgs a) {
     agg_tile<C, RAW, NM>(a, blockIdx.x, blockIdx.y, blockIdx.z, a.N);
 }
 
 // Persistent form: one tile per take (the workgroup is the unit); item ->
 // (job, frame, tile row, tile column).
 template <int C, int NM>
-__global__ __launch_bounds__(256) void k_aggregate_queue(const AArgs* __restrict__ jobs, WorkQueue q, int nl) {
+__global__ __launch_bounds__(kAggThreads) void k_aggregate_queue(const AArgs* __restrict__ jobs, WorkQueue q, int nl) {
     __shared__ long s_item;
     for (;;) {
         __syncthreads();  // every thread has read the previous item
@@ -1759,7 +1765,7 @@ __global__ __launch_bounds__(256) void k_aggregate_queue(const AArgs* __restrict
         typedef const __attribute__((address_space(4))) AArgs cAA;
         cAA& a = ((cAA*)jobs)[j];  // scalar loads of the job
         const long local = it - q.item_begin[j];
-        const long tx = (a.W + kAggTile - 1) / kAggTile, ty = (a.H + kAggTile - 1) / kAggTile;
+        const long tx = (a.W + kAggTileX - 1) / kAggTileX, ty = (a.H + kAggTileY - 1) / kAggTileY;
         const long bz = local / (tx * ty), rem = local - bz * tx * ty;
         agg_tile<C, false, NM>(a, (int)(rem % tx), (int)(rem / tx), (unsigned)bz, nl);
     }
@@ -2040,14 +2046,15 @@ hipError_t launch_filter_queue(int K, int C, int step, const FArgs* jobs, const 
 hipError_t launch_aggregate_queue(int C, const AArgs* jobs, const WorkQueue& q, int blocks, int maxN, hipStream_t s) {
     if (q.total <= 0 || blocks <= 0) return hipSuccess;
     if (maxN < 1 || maxN > kMaxN) return hipErrorInvalidValue;
-    const size_t lds = (size_t)2 * 256 * maxN * sizeof(uint32_t);
+    const size_t lds = (size_t)2 * kAggThreads * maxN * sizeof(uint32_t);
+    const dim3 blk(kAggThreads);
     // members held in registers by the scan: 8, or 16 when some job has N = 16
     if (maxN <= 8) {
-        if (C == 1) hipLaunchKernelGGL((k_aggregate_queue<1, 8>), dim3(blocks), dim3(256), lds, s, jobs, q, maxN);
-        else hipLaunchKernelGGL((k_aggregate_queue<3, 8>), dim3(blocks), dim3(256), lds, s, jobs, q, maxN);
+        if (C == 1) hipLaunchKernelGGL((k_aggregate_queue<1, 8>), dim3(blocks), blk, lds, s, jobs, q, maxN);
+        else hipLaunchKernelGGL((k_aggregate_queue<3, 8>), dim3(blocks), blk, lds, s, jobs, q, maxN);
     } else {
-        if (C == 1) hipLaunchKernelGGL((k_aggregate_queue<1, 16>), dim3(blocks), dim3(256), lds, s, jobs, q, maxN);
-        else hipLaunchKernelGGL((k_aggregate_queue<3, 16>), dim3(blocks), dim3(256), lds, s, jobs, q, maxN);
+        if (C == 1) hipLaunchKernelGGL((k_aggregate_queue<1, 16>), dim3(blocks), blk, lds, s, jobs, q, maxN);
+        else hipLaunchKernelGGL((k_aggregate_queue<3, 16>), dim3(blocks), blk, lds, s, jobs, q, maxN);
     }
     return hipGetLastError();
 }
@@ -2063,17 +2070,18 @@ int filter_words(int K, int C, int N, int step) { return filter_lds_words(K, C, 
 
 hipError_t launch_aggregate(int C, const AArgs& a, unsigned frames, hipStream_t s) {
     if (frames == 0) return hipSuccess;
-    dim3 grid((a.W + kAggTile - 1) / kAggTile, (a.H + kAggTile - 1) / kAggTile, frames);
+    dim3 grid((a.W + kAggTileX - 1) / kAggTileX, (a.H + kAggTileY - 1) / kAggTileY, frames);
     if (a.N < 1 || a.N > kMaxN) return hipErrorInvalidValue;
-    const size_t lds = (size_t)2 * 256 * a.N * sizeof(uint32_t);  // the chunk's ordered list
+    const size_t lds = (size_t)2 * kAggThreads * a.N * sizeof(uint32_t);  // the chunk's ordered list
+    const dim3 blk(kAggThreads);
     // members held in registers by the scan: 8, or 16 for N = 16
-#define VB_AGG(NM)                                                                               \
-    if (a.raw) {                                                                                 \
-        if (C == 1) hipLaunchKernelGGL((k_aggregate<1, true, NM>), grid, dim3(256), lds, s, a);  \
-        else hipLaunchKernelGGL((k_aggregate<3, true, NM>), grid, dim3(256), lds, s, a);         \
-    } else {                                                                                     \
-        if (C == 1) hipLaunchKernelGGL((k_aggregate<1, false, NM>), grid, dim3(256), lds, s, a); \
-        else hipLaunchKernelGGL((k_aggregate<3, false, NM>), grid, dim3(256), lds, s, a);        \
+#define VB_AGG(NM)                                                                         \
+    if (a.raw) {                                                                           \
+        if (C == 1) hipLaunchKernelGGL((k_aggregate<1, true, NM>), grid, blk, lds, s, a);  \
+        else hipLaunchKernelGGL((k_aggregate<3, true, NM>), grid, blk, lds, s, a);         \
+    } else {                                                                               \
+        if (C == 1) hipLaunchKernelGGL((k_aggregate<1, false, NM>), grid, blk, lds, s, a); \
+        else hipLaunchKernelGGL((k_aggregate<3, false, NM>), grid, blk, lds, s, a);        \
     }
     if (a.N <= 8) {
         VB_AGG(8)
