@@ -210,6 +210,24 @@ def test_run_rgb_no_colour_and_hadamard(small_rgb, oracle, gpu_ctx):
     _run_both(noisy, 20.0, oracle, gpu_ctx, p1=p1, p2=p2)
 
 
+@pytest.mark.parametrize("rgb", [False, True])
+def test_run_group_size_16(cfg1, small_rgb, oracle, gpu_ctx, rgb):
+    """N = 16, the largest group the kernels hold: most groups have 16
+    members, so the filter's 16-member Haar, the aggregation's 16-member
+    scan and its largest chunk list all run.  On the 7-frame RGB clip
+    Nb = 4 gives enough candidates (and the general matching form)."""
+    import vbm3d_amd as V
+    _, noisy = small_rgb if rgb else cfg1
+    nb = 4 if rgb else V.NONE
+    p1 = V.initialize_parameters_1(20.0, N=16, Nb=nb)
+    p2 = V.initialize_parameters_2(20.0, N=16, Nb=nb)
+    _run_both(noisy, 20.0, oracle, gpu_ctx, p1=p1, p2=p2)
+    vid = oracle.color(noisy, True) if rgb else noisy
+    F, C, H, W = noisy.shape
+    _, _, _, nsx = V.block_match(vid, p1, all_refs(W, H, F, C, p1.k, p1.p), ctx=gpu_ctx)
+    assert (nsx == 16).mean() > 0.3
+
+
 def test_run_chunked_ring(cfg1, oracle, gpu_ctx, monkeypatch):
     """A tiny estimate budget forces one-frame chunks and a ring of
     2 Nf + 1 frames; the result must not change."""
