@@ -228,6 +228,27 @@ def test_run_group_size_16(cfg1, small_rgb, oracle, gpu_ctx, rgb):
     assert (nsx == 16).mean() > 0.3
 
 
+@pytest.mark.parametrize("shape", [(9, 9, 1, 1), (12, 9, 2, 3), (33, 17, 3, 1), (40, 8, 5, 3), (9, 40, 1, 3)])
+def test_run_edge_sizes(oracle, gpu_ctx, shape):
+    """Frames one pixel larger than the step-1 patch, a frame exactly one
+    patch high, one-frame clips, and sizes that leave partial aggregation
+    tiles on both axes: the oracle's bits."""
+    W, H, F, C = shape
+    _, noisy = synth.noisy_clip(W, H, F, C, 20.0, 5)
+    _run_both(noisy, 20.0, oracle, gpu_ctx)
+
+
+def test_run_rejects_window_smaller_than_nb(oracle, gpu_ctx):
+    """An 8x8 frame leaves one step-1 candidate for Nb = 2 matches, which
+    the reference's partial_sort cannot take; both paths refuse it."""
+    import vbm3d_amd as V
+    _, noisy = synth.noisy_clip(8, 8, 2, 1, 20.0, 5)
+    with pytest.raises(V.VBM3DError, match="smaller than Nb"):
+        V.denoise(noisy, 20.0, ctx=gpu_ctx)
+    with pytest.raises(RuntimeError, match="smaller than Nb"):
+        oracle.run_vbm3d(20.0, noisy, oracle.params_1(20.0), oracle.params_2(20.0), True, 1)
+
+
 def test_run_chunked_ring(cfg1, oracle, gpu_ctx, monkeypatch):
     """A tiny estimate budget forces one-frame chunks and a ring of
     2 Nf + 1 frames; the result must not change."""
