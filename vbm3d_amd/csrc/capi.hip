@@ -251,6 +251,11 @@ struct vbm3d_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     DevBuf noisy, basic, den, ff, bf, num_raw, den_raw;  // host-buffer entry points
+    // vbm3d_run: colour round trips of the finished noisy and basic videos,
+    // copied to the host on the side stream while the later steps compute
+    DevBuf noisy_rt, basic_rt;
+    hipStream_t side = nullptr;
+    hipEvent_t ready[2] = {nullptr, nullptr};
     DevBuf idx, nsx, nsim, dist, est, wgt, tabs, refxy, meta, gpos, rlist, scal, stats, counts;
     size_t est_budget = 0;  // bytes for the per-patch rings; 0 = from free device memory
     unsigned chunk_frames = 0;  // per-clip path: frames per chunk; 0 = from the budget
@@ -467,9 +472,23 @@ int flow_absmax(vbm3d_ctx* ctx, const float* ff, const float* bf, unsigned W, un
 }
 
 // run_vbm3d on device buffers (src/vbm3d.cpp:87-376).
+// Early outputs of run_device for the host-buffer entry.  The noisy video
+// is final once the forward colour transform has run (no step writes it),
+// the basic estimate once step 1 has run; each is then copied, given the
+// inverse colour transform the end of the run would apply, and `ready`
+// recorded, so that its copy to the host can overlap the rest of the run.
+// Without the colour round trip nothing is copied: the event marks the
+// source buffer itself as final.
+struct EarlyOut {
+    float* noisy_rt;  // null without the round trip
+    float* basic_rt;
+    hipEvent_t noisy_ready, basic_ready;
+};
+
 int run_device(vbm3d_ctx* ctx, float sigma, float* noisy, const float* ff, const float* bf,
                unsigned W, unsigned H, unsigned F, unsigned C, const vbm3d_params* p1,
-               const vbm3d_params* p2, int cs, int nb, float* basic, float* den, hipStream_t s) {
+               const vbm3d_params* p2, int cs, int nb, float* basic, float* den, hipStream_t s,
+               const EarlyOut* eo = nullptr) {
     if (!ctx || !noisy || !basic || !den || !p1 || !p2) return fail(VBM3D_ERR_ARG, "null argument");
     if (C != 1 && C != 3) return fail(VBM3D_ERR_ARG, "only 1 or 3 channels are supported");
     if (p1->k > 0) { int rc = check(*p1, W, H, F, C, 1); if (rc) return rc; }
@@ -479,6 +498,16 @@ int run_device(vbm3d_ctx* ctx, float sigma, float* noisy, const float* ff, const
     if (rc) return rc;
     if (cs && C == 3)
         HIP_TRY(timed(ctx, 3, 0, s, [&] { return launch_color(p1->k > 0 ? noisy : basic, W, H, F, true, s); }));
+    const bool rt = cs && C == 3;
+    auto early = [&](const float* src, float* dst, hipEvent_t ev) -> hipError_t {
+        hipError_t e = hipSuccess;
+        if (rt) {
+            e = hipMemcpyAsync(dst, src, (size_t)W * H * F * C * 4, hipMemcpyDeviceToDevice, s);
+            if (e == hipSuccess) e = timed(ctx, 3, 0, s, [&] { return launch_color(dst, W, H, F, false, s); });
+        }
+        return e == hipSuccess ? hipEventRecord(ev, s) : e;
+    };
+    if (eo) HIP_TRY(early(noisy, eo->noisy_rt, eo->noisy_ready));
     StepIO io{};
     io.sigma = sigma; io.noisy = noisy; io.ff = ff; io.bf = bf;
     io.W = W; io.H = H; io.F = F; io.C = C; io.cs = cs != 0; io.nb = nb; io.maxflow = maxflow;
@@ -488,6 +517,7 @@ int run_device(vbm3d_ctx* ctx, float sigma, float* noisy, const float* ff, const
         rc = run_step(ctx, io, s);
         if (rc) return rc;
     }
+    if (eo) HIP_TRY(early(basic, eo->basic_rt, eo->basic_ready));
     if (p2->k > 0) {
         io.step = 2; io.p = p2; io.basic = basic;
         io.out = den; io.fallback = nb == 1 ? basic : noisy;  // (:200, :343)
@@ -498,10 +528,12 @@ int run_device(vbm3d_ctx* ctx, float sigma, float* noisy, const float* ff, const
     }
     if (cs && C == 3) {
         HIP_TRY(timed(ctx, 3, 0, s, [&] { return launch_color(den, W, H, F, false, s); }));
-        HIP_TRY(timed(ctx, 3, 0, s, [&] { return launch_color(noisy, W, H, F, false, s); }));
-        HIP_TRY(timed(ctx, 3, 0, s, [&] { return launch_color(basic, W, H, F, false, s); }));
+        if (!eo) {
+            HIP_TRY(timed(ctx, 3, 0, s, [&] { return launch_color(noisy, W, H, F, false, s); }));
+            HIP_TRY(timed(ctx, 3, 0, s, [&] { return launch_color(basic, W, H, F, false, s); }));
+        }
     }
-    HIP_TRY(hipStreamSynchronize(s));
+    if (!eo) HIP_TRY(hipStreamSynchronize(s));  // else the caller synchronises
     return VBM3D_OK;
 }
 
@@ -980,7 +1012,8 @@ void vbm3d_destroy(vbm3d_ctx* c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
-    for (DevBuf* b : {&c->noisy, &c->basic, &c->den, &c->ff, &c->bf, &c->num_raw, &c->den_raw,
+    if (c->side) (void)hipStreamSynchronize(c->side);
+    for (DevBuf* b : {&c->noisy, &c->basic, &c->den, &c->ff, &c->bf, &c->num_raw, &c->den_raw, &c->noisy_rt, &c->basic_rt,
                       &c->idx, &c->nsx, &c->nsim, &c->dist, &c->est, &c->wgt, &c->tabs,
                       &c->refxy, &c->meta, &c->gpos, &c->rlist, &c->scal, &c->stats, &c->counts,
                       &c->b_idx, &c->b_nsx, &c->b_est, &c->b_wgt, &c->b_refxy, &c->b_meta, &c->b_gpos,
@@ -988,6 +1021,9 @@ void vbm3d_destroy(vbm3d_ctx* c) {
         if (b->p) (void)hipFree(b->p);
     if (c->b_exec) (void)hipGraphExecDestroy(c->b_exec);
     if (c->b_graph) (void)hipGraphDestroy(c->b_graph);
+    for (hipEvent_t e : c->ready)
+        if (e) (void)hipEventDestroy(e);
+    if (c->side) (void)hipStreamDestroy(c->side);
     (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -1074,13 +1110,39 @@ int vbm3d_run(vbm3d_ctx* ctx, float sigma, float* noisy, const float* fflow, con
     HIP_TRY(ensure(ctx->den, n * 4));
     if (fflow) HIP_TRY(upload(ctx->ff, fflow, nf, s));
     if (bflow) HIP_TRY(upload(ctx->bf, bflow, nf, s));
+    // The noisy video changes only through the colour round trip; the
+    // noisy and basic videos go to the host on the side stream as soon as
+    // they are final, while the later steps compute.
+    const bool rt = color_space && C == 3;
+    if (rt) {
+        HIP_TRY(ensure(ctx->noisy_rt, n * 4));
+        HIP_TRY(ensure(ctx->basic_rt, n * 4));
+    }
+    if (!ctx->side) HIP_TRY(hipStreamCreate(&ctx->side));
+    for (hipEvent_t& e : ctx->ready)
+        if (!e) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    const EarlyOut eo{rt ? (float*)ctx->noisy_rt.p : nullptr, rt ? (float*)ctx->basic_rt.p : nullptr,
+                      ctx->ready[0], ctx->ready[1]};
     int rc = run_device(ctx, sigma, (float*)ctx->noisy.p, fflow ? (float*)ctx->ff.p : nullptr,
                         bflow ? (float*)ctx->bf.p : nullptr, W, H, F, C, p1, p2, color_space,
-                        ref_grid_nb, (float*)ctx->basic.p, (float*)ctx->den.p, s);
-    if (rc) return rc;
-    HIP_TRY(hipMemcpy(noisy, ctx->noisy.p, n * 4, hipMemcpyDeviceToHost));
-    HIP_TRY(hipMemcpy(basic, ctx->basic.p, n * 4, hipMemcpyDeviceToHost));
-    HIP_TRY(hipMemcpy(denoised, ctx->den.p, n * 4, hipMemcpyDeviceToHost));
+                        ref_grid_nb, (float*)ctx->basic.p, (float*)ctx->den.p, s, &eo);
+    if (rc) {
+        (void)hipStreamSynchronize(s);
+        return rc;
+    }
+    hipError_t e = hipSuccess;
+    if (rt) {
+        e = hipStreamWaitEvent(ctx->side, eo.noisy_ready, 0);
+        if (e == hipSuccess) e = hipMemcpyAsync(noisy, eo.noisy_rt, n * 4, hipMemcpyDeviceToHost, ctx->side);
+    }
+    if (e == hipSuccess) e = hipStreamWaitEvent(ctx->side, eo.basic_ready, 0);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(basic, rt ? eo.basic_rt : ctx->basic.p, n * 4, hipMemcpyDeviceToHost, ctx->side);
+    if (e == hipSuccess) e = hipMemcpyAsync(denoised, ctx->den.p, n * 4, hipMemcpyDeviceToHost, s);
+    const hipError_t e1 = hipStreamSynchronize(ctx->side), e2 = hipStreamSynchronize(s);
+    HIP_TRY(e);
+    HIP_TRY(e1);
+    HIP_TRY(e2);
     return VBM3D_OK;
 }
 
