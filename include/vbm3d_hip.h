@@ -164,8 +164,9 @@ typedef struct vbm3d_batch_config {
     int grab;               /* reference patches a wave takes from the work queue at a time, >= 1 */
     int groups_per_cu;      /* persistent workgroups per compute unit, >= 1 */
     int use_graph;          /* capture the whole batch in a HIP graph and replay it on later identical calls */
-    int persistent;         /* 1: matching and filtering run as persistent workgroups that take patches from
-                               the global work queue; 0: one wave per patch, in workgroup dispatch order */
+    int persistent;         /* 1: matching, filtering and aggregation run as persistent workgroups that take
+                               patches (output tiles) from the global work queue; 0: one wave per patch and
+                               one workgroup per output tile, in workgroup dispatch order */
 } vbm3d_batch_config;
 
 /* Denoises `nclips` clips of any sizes and parameters together: per step,

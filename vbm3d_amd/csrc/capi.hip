@@ -929,8 +929,12 @@ int exec_batch(vbm3d_ctx* ctx, const BatchPlan& plan, const vbm3d_batch_config& 
             q.total = op.total;
             q.grab = op.kind == 2 ? 1 : std::max(1, cfg.grab);
             if (op.kind == 2) {
-                const long blocks = std::min<long>(op.total, (long)cfg.groups_per_cu * cus);
-                HIP_TRY(launch_aggregate_queue(op.C, (const AArgs*)(blob + op.jobs), q, (int)blocks, op.shape, s));
+                if (cfg.persistent) {
+                    const long blocks = std::min<long>(op.total, (long)cfg.groups_per_cu * cus);
+                    HIP_TRY(launch_aggregate_queue(op.C, (const AArgs*)(blob + op.jobs), q, (int)blocks, op.shape, s));
+                } else {
+                    HIP_TRY(launch_aggregate_multi(op.C, (const AArgs*)(blob + op.jobs), q, op.shape, s));
+                }
                 continue;
             }
             int waves = cfg.waves_per_group > 0 ? std::min(cfg.waves_per_group, 4) : 4;

@@ -191,6 +191,7 @@ hipError_t launch_bm_multi(int K, int C, const BMArgs* jobs, const WorkQueue& q,
 hipError_t launch_filter_multi(int K, int C, int step, const FArgs* jobs, const WorkQueue& q, int words,
                                int waves, hipStream_t s);
 hipError_t launch_aggregate_queue(int C, const AArgs* jobs, const WorkQueue& q, int blocks, int maxN, hipStream_t s);
+hipError_t launch_aggregate_multi(int C, const AArgs* jobs, const WorkQueue& q, int maxN, hipStream_t s);
 hipError_t launch_zero_u64(unsigned long long* p, int n, hipStream_t s);
 int bm_lds_words(int K, int C, int Ns, int Npr, int Nb, int Nf);  // per wave, 4-byte words
 int filter_words(int K, int C, int N, int step);

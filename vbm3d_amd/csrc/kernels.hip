@@ -1750,6 +1750,23 @@ __global__ __launch_bounds__(kAggThreads) __attribute__((amdgpu_waves_per_eu(8))
     agg_tile<C, RAW, NM>(a, blockIdx.x, blockIdx.y, blockIdx.z, a.N);
 }
 
+// Batched form with one workgroup per tile: workgroup b takes item b, so
+// the hardware's dispatch order is the queue; item -> (job, frame, tile
+// row, tile column).  The same occupancy floor as k_aggregate.
+template <int C, int NM>
+__global__ __launch_bounds__(kAggThreads) __attribute__((amdgpu_waves_per_eu(8))) void k_aggregate_multi(
+    const AArgs* __restrict__ jobs, WorkQueue q, int nl) {
+    const long it = (long)blockIdx.x;
+    if (it >= q.total) return;  // workgroup-uniform
+    const int j = queue_job(q, it);
+    typedef const __attribute__((address_space(4))) AArgs cAA;
+    cAA& a = ((cAA*)jobs)[j];  // scalar loads of the job
+    const long local = it - uniform_long(q.item_begin[j]);
+    const long tx = (a.W + kAggTileX - 1) / kAggTileX, ty = (a.H + kAggTileY - 1) / kAggTileY;
+    const long bz = local / (tx * ty), rem = local - bz * tx * ty;
+    agg_tile<C, false, NM>(a, (int)(rem % tx), (int)(rem / tx), (unsigned)bz, nl);
+}
+
 // Persistent form: one tile per take (the workgroup is the unit); item ->
 // (job, frame, tile row, tile column).
 template <int C, int NM>
@@ -2041,6 +2058,21 @@ hipError_t launch_filter_queue(int K, int C, int step, const FArgs* jobs, const 
         case 8: return filter_queue_kc<8>(C, step, jobs, q, words, waves, blocks, s);
         default: return hipErrorInvalidValue;
     }
+}
+
+hipError_t launch_aggregate_multi(int C, const AArgs* jobs, const WorkQueue& q, int maxN, hipStream_t s) {
+    if (q.total <= 0) return hipSuccess;
+    if (maxN < 1 || maxN > kMaxN || q.total > 0x7FFFFFFFL) return hipErrorInvalidValue;
+    const size_t lds = (size_t)2 * kAggThreads * maxN * sizeof(uint32_t);
+    const dim3 grid((unsigned)q.total), blk(kAggThreads);
+    if (maxN <= 8) {
+        if (C == 1) hipLaunchKernelGGL((k_aggregate_multi<1, 8>), grid, blk, lds, s, jobs, q, maxN);
+        else hipLaunchKernelGGL((k_aggregate_multi<3, 8>), grid, blk, lds, s, jobs, q, maxN);
+    } else {
+        if (C == 1) hipLaunchKernelGGL((k_aggregate_multi<1, 16>), grid, blk, lds, s, jobs, q, maxN);
+        else hipLaunchKernelGGL((k_aggregate_multi<3, 16>), grid, blk, lds, s, jobs, q, maxN);
+    }
+    return hipGetLastError();
 }
 
 hipError_t launch_aggregate_queue(int C, const AArgs* jobs, const WorkQueue& q, int blocks, int maxN, hipStream_t s) {
