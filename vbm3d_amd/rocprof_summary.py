@@ -18,6 +18,8 @@ bench.py reads for ``roofline.traffic``.
 
     python -m vbm3d_amd.rocprof_summary --stats S.csv --fetch F.csv --write W.csv \\
         --copy-bytes N --workload "..." --md out.md --json profiles/traffic.json
+
+Without --fetch and --write it tabulates the kernel statistics alone.
 """
 from __future__ import annotations
 
@@ -76,17 +78,31 @@ def per_kernel(rows, factor):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--stats", required=True)
-    ap.add_argument("--fetch", required=True)
-    ap.add_argument("--write", required=True)
-    ap.add_argument("--copy-bytes", type=float, required=True)
+    ap.add_argument("--fetch")
+    ap.add_argument("--write")
+    ap.add_argument("--copy-bytes", type=float)
     ap.add_argument("--workload", required=True)
     ap.add_argument("--md", required=True)
-    ap.add_argument("--json", required=True)
+    ap.add_argument("--json")
     ap.add_argument("--commands", nargs="*", default=[])
     ap.add_argument("--min-ms", type=float, default=0.5, help="list kernels above this average")
     args = ap.parse_args()
 
     stats = read_stats(args.stats)
+    if not (args.fetch and args.write):
+        lines = [f"# rocprofv3 kernel statistics: {args.workload}", "",
+                 "Commands (run from the repository root):", ""]
+        lines += [f"    {c}" for c in args.commands]
+        lines += ["", "| kernel | calls | avg ms | total ms |", "|---|---|---|---|"]
+        for k, s in sorted(stats.items(), key=lambda kv: -kv[1]["total_ms"]):
+            if s["total_ms"] >= args.min_ms:
+                lines.append(f"| `{k}` | {s['calls']} | {s['avg_ms']:.3f} | {s['total_ms']:.1f} |")
+        with open(args.md, "w") as f:
+            f.write("\n".join(lines) + "\n")
+        print("\n".join(lines))
+        return
+    if args.copy_bytes is None or not args.json:
+        raise SystemExit("--fetch and --write need --copy-bytes and --json")
     fe = read_counter(args.fetch, "FETCH_SIZE")
     wr = read_counter(args.write, "WRITE_SIZE")
     ff, fw = calibrate(fe, args.copy_bytes), calibrate(wr, args.copy_bytes)
