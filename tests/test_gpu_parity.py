@@ -249,6 +249,30 @@ def test_run_rejects_window_smaller_than_nb(oracle, gpu_ctx):
         oracle.run_vbm3d(20.0, noisy, oracle.params_1(20.0), oracle.params_2(20.0), True, 1)
 
 
+@pytest.mark.parametrize("colour", [True, False])
+def test_run_skipped_steps(small_rgb, oracle, gpu_ctx, colour):
+    """The host-buffer entry with step 1 skipped (k = 0: the basic estimate
+    is an input, and takes the colour round trip) and with step 2 skipped
+    (the final estimate is the basic one): every returned video, the noisy
+    one after its round trip included, equals the oracle's."""
+    import vbm3d_amd as V
+    _, noisy = small_rgb
+    p1, p2 = V.initialize_parameters_1(20.0), V.initialize_parameters_2(20.0)
+    skip1, skip2 = V.initialize_parameters_1(20.0), V.initialize_parameters_2(20.0)
+    skip1.k = 0
+    skip2.k = 0
+    given = noisy[::-1].copy()  # any video of the right shape serves as the basic estimate
+    for q1, q2, basic_in in [(skip1, p2, given), (p1, skip2, None)]:
+        nz = noisy.copy()
+        gb = np.zeros_like(nz) if basic_in is None else basic_in.copy()
+        gf = np.zeros_like(nz)
+        V.run_vbm3d(20.0, nz, None, None, gb, gf, q1, q2, colour, 1, ctx=gpu_ctx)
+        ob, of, onz = oracle.run_vbm3d(20.0, noisy, q1, q2, colour, 1, basic_in=basic_in)
+        assert_bits_equal(gb, ob, "basic")
+        assert_bits_equal(gf, of, "final")
+        assert_bits_equal(nz, onz, "noisy after the colour round trip")
+
+
 def test_run_chunked_ring(cfg1, oracle, gpu_ctx, monkeypatch):
     """A tiny estimate budget forces one-frame chunks and a ring of
     2 Nf + 1 frames; the result must not change."""
