@@ -83,9 +83,10 @@ void vbm3d_default_params_2(vbm3d_params* p, int k, int kt, int Nf, int Ns,
  *   - ref_grid_nb = 1 gives the single-thread reference grid, 2..32
  *     (power of two) the grid and merge order of the reference run with
  *     that many OpenMP threads (src/vbm3d.cpp:117-128, 213-354).
- * The noisy and basic videos are copied back on a second stream of the
- * context while the later steps compute; the call returns when all three
- * videos are in host memory. */
+ * The noisy video is uploaded in chunks on a second stream of the context
+ * while step 1 runs on the frames already there, and the noisy and basic
+ * videos are copied back on that stream while the later steps compute;
+ * the call returns when all three videos are in host memory. */
 int vbm3d_run(vbm3d_ctx* ctx, float sigma, float* noisy, const float* fflow,
               const float* bflow, unsigned W, unsigned H, unsigned F,
               unsigned C, const vbm3d_params* p1, const vbm3d_params* p2,

@@ -16,6 +16,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -255,7 +256,7 @@ struct vbm3d_ctx {
     // copied to the host on the side stream while the later steps compute
     DevBuf noisy_rt, basic_rt;
     hipStream_t side = nullptr;
-    hipEvent_t ready[2] = {nullptr, nullptr};
+    hipEvent_t ready[3] = {nullptr, nullptr, nullptr};  // noisy final, basic final, upload chunk
     DevBuf idx, nsx, nsim, dist, est, wgt, tabs, refxy, meta, gpos, rlist, scal, stats, counts;
     size_t est_budget = 0;  // bytes for the per-patch rings; 0 = from free device memory
     unsigned chunk_frames = 0;  // per-clip path: frames per chunk; 0 = from the budget
@@ -296,6 +297,11 @@ struct StepIO {
     bool adjoint;          // step-2 adjoint: num_raw receives J^T gy
     const float* gy;
     const float* den;      // the forward pass's denominator
+    // Optional: called before each chunk's launches with the number of
+    // leading frames the chunk reads (the host-buffer entry uploads the
+    // video while step 1 runs); max_chunk caps the chunk (0 = no cap).
+    std::function<int(unsigned)> need_frames;
+    unsigned max_chunk = 0;
 };
 
 // Brackets one launch with events when profiling is on.
@@ -352,6 +358,7 @@ int run_step(vbm3d_ctx* ctx, const StepIO& io, hipStream_t s) {
     size_t cap_frames = std::max<size_t>(1, budget / std::max<size_t>(per_frame, 1));
     unsigned S = (unsigned)std::max<long>(1, (long)cap_frames - 2 * (long)p.Nf);
     if (ctx->chunk_frames) S = std::min(S, ctx->chunk_frames);
+    if (io.max_chunk) S = std::min(S, io.max_chunk);
     unsigned ring = std::min<unsigned>(Fr, S + 2 * p.Nf);
     if (S >= Fr) { S = Fr; ring = Fr; }
     const size_t slots = (size_t)ring * nrf;
@@ -438,6 +445,11 @@ int run_step(vbm3d_ctx* ctx, const StepIO& io, hipStream_t s) {
         const unsigned e = std::min(Fr, a + S);
         bm.r_begin = fa.r_begin = (long)a * nrf;
         bm.r_end = fa.r_end = (long)e * nrf;
+        // matching and filtering read frames up to e - 1 + Nf
+        if (io.need_frames) {
+            rc = io.need_frames(std::min(io.F, e + p.Nf + p.kt - 1));
+            if (rc) return rc;
+        }
         HIP_TRY(timed(ctx, 0, io.step, s, [&] { return launch_bm((int)K, bm, s); }));
         if (ctx->count)
             HIP_TRY(launch_sum_counts(bm.count_out, bm.r_end - bm.r_begin,
@@ -485,10 +497,21 @@ struct EarlyOut {
     hipEvent_t noisy_ready, basic_ready;
 };
 
+// Upload of the noisy video by run_device itself for the host-buffer
+// entry: `chunk` frames per copy on the side stream, each followed by its
+// forward colour transform on the main stream, issued as step 1 (run in
+// chunks of `step1_chunk` frames) reaches frames it has not read yet.
+struct HostUpload {
+    const float* src;  // the caller's video [F][C][H][W]
+    unsigned chunk, step1_chunk;
+    hipStream_t side;
+    hipEvent_t ev;
+};
+
 int run_device(vbm3d_ctx* ctx, float sigma, float* noisy, const float* ff, const float* bf,
                unsigned W, unsigned H, unsigned F, unsigned C, const vbm3d_params* p1,
                const vbm3d_params* p2, int cs, int nb, float* basic, float* den, hipStream_t s,
-               const EarlyOut* eo = nullptr) {
+               const EarlyOut* eo = nullptr, const HostUpload* hu = nullptr) {
     if (!ctx || !noisy || !basic || !den || !p1 || !p2) return fail(VBM3D_ERR_ARG, "null argument");
     if (C != 1 && C != 3) return fail(VBM3D_ERR_ARG, "only 1 or 3 channels are supported");
     if (p1->k > 0) { int rc = check(*p1, W, H, F, C, 1); if (rc) return rc; }
@@ -496,8 +519,24 @@ int run_device(vbm3d_ctx* ctx, float sigma, float* noisy, const float* ff, const
     unsigned maxflow = 0;
     int rc = flow_absmax(ctx, ff, bf, W, H, F, s, &maxflow);
     if (rc) return rc;
-    if (cs && C == 3)
+    if (hu && p1->k == 0) return fail(VBM3D_ERR_ARG, "internal: chunked upload needs step 1");
+    if (cs && C == 3 && !hu)
         HIP_TRY(timed(ctx, 3, 0, s, [&] { return launch_color(p1->k > 0 ? noisy : basic, W, H, F, true, s); }));
+    // chunked upload: frames [0, up) are on the device and colour-transformed
+    unsigned up = 0;
+    auto need = [&](unsigned fr) -> int {
+        while (up < fr) {
+            const unsigned n1 = std::min(F - up, hu->chunk);
+            const size_t off = (size_t)up * W * H * C, cnt = (size_t)n1 * W * H * C;
+            HIP_TRY(hipMemcpyAsync(noisy + off, hu->src + off, cnt * 4, hipMemcpyHostToDevice, hu->side));
+            HIP_TRY(hipEventRecord(hu->ev, hu->side));
+            HIP_TRY(hipStreamWaitEvent(s, hu->ev, 0));
+            if (cs && C == 3)
+                HIP_TRY(timed(ctx, 3, 0, s, [&] { return launch_color(noisy + off, W, H, n1, true, s); }));
+            up += n1;
+        }
+        return VBM3D_OK;
+    };
     const bool rt = cs && C == 3;
     auto early = [&](const float* src, float* dst, hipEvent_t ev) -> hipError_t {
         hipError_t e = hipSuccess;
@@ -507,15 +546,26 @@ int run_device(vbm3d_ctx* ctx, float sigma, float* noisy, const float* ff, const
         }
         return e == hipSuccess ? hipEventRecord(ev, s) : e;
     };
-    if (eo) HIP_TRY(early(noisy, eo->noisy_rt, eo->noisy_ready));
+    if (eo && !hu) HIP_TRY(early(noisy, eo->noisy_rt, eo->noisy_ready));
     StepIO io{};
     io.sigma = sigma; io.noisy = noisy; io.ff = ff; io.bf = bf;
     io.W = W; io.H = H; io.F = F; io.C = C; io.cs = cs != 0; io.nb = nb; io.maxflow = maxflow;
     if (p1->k > 0) {
         io.step = 1; io.p = p1; io.basic = nullptr;
         io.out = basic; io.fallback = noisy;  // den == 0 -> noisy (:169, :277)
+        if (hu) {
+            io.need_frames = need;
+            io.max_chunk = hu->step1_chunk;
+        }
         rc = run_step(ctx, io, s);
+        io.need_frames = nullptr;
+        io.max_chunk = 0;
         if (rc) return rc;
+    }
+    if (hu) {
+        rc = need(F);  // every frame, whatever step 1 read
+        if (rc) return rc;
+        if (eo) HIP_TRY(early(noisy, eo->noisy_rt, eo->noisy_ready));
     }
     if (eo) HIP_TRY(early(basic, eo->basic_rt, eo->basic_ready));
     if (p2->k > 0) {
@@ -1109,7 +1159,11 @@ int vbm3d_run(vbm3d_ctx* ctx, float sigma, float* noisy, const float* fflow, con
     HIP_TRY(hipSetDevice(ctx->device));
     hipStream_t s = ctx->stream;
     const size_t n = (size_t)W * H * F * C, nf = (size_t)(F > 1 ? F - 1 : 1) * W * H * 2;
-    HIP_TRY(upload(ctx->noisy, noisy, n, s));
+    // With step 1 on, the noisy video is uploaded in chunks while step 1
+    // runs on the frames already there (run_device's HostUpload).
+    const bool chunked = p1->k > 0;
+    if (chunked) HIP_TRY(ensure(ctx->noisy, n * 4));
+    else HIP_TRY(upload(ctx->noisy, noisy, n, s));
     HIP_TRY(upload(ctx->basic, p1->k > 0 ? nullptr : basic, n, s));
     HIP_TRY(ensure(ctx->den, n * 4));
     if (fflow) HIP_TRY(upload(ctx->ff, fflow, nf, s));
@@ -1127,10 +1181,14 @@ int vbm3d_run(vbm3d_ctx* ctx, float sigma, float* noisy, const float* fflow, con
         if (!e) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     const EarlyOut eo{rt ? (float*)ctx->noisy_rt.p : nullptr, rt ? (float*)ctx->basic_rt.p : nullptr,
                       ctx->ready[0], ctx->ready[1]};
+    // 4 frames per copy; step 1 in chunks of 8 reference frames, so that
+    // its first launches wait for 8 + Nf frames only
+    const HostUpload hu{noisy, 4, 8, ctx->side, ctx->ready[2]};
     int rc = run_device(ctx, sigma, (float*)ctx->noisy.p, fflow ? (float*)ctx->ff.p : nullptr,
                         bflow ? (float*)ctx->bf.p : nullptr, W, H, F, C, p1, p2, color_space,
-                        ref_grid_nb, (float*)ctx->basic.p, (float*)ctx->den.p, s, &eo);
+                        ref_grid_nb, (float*)ctx->basic.p, (float*)ctx->den.p, s, &eo, chunked ? &hu : nullptr);
     if (rc) {
+        (void)hipStreamSynchronize(ctx->side);
         (void)hipStreamSynchronize(s);
         return rc;
     }
