@@ -273,6 +273,28 @@ def test_run_skipped_steps(small_rgb, oracle, gpu_ctx, colour):
         assert_bits_equal(nz, onz, "noisy after the colour round trip")
 
 
+def test_run_with_flow(oracle, gpu_ctx):
+    """The whole path with forward and backward flows, through the
+    host-buffer entry: 20 RGB frames, so that step 1 runs in several chunks
+    while the upload is still arriving, and the aggregation's reach grows
+    with the largest flow."""
+    import vbm3d_amd as V
+    W, H, F = 80, 56, 20
+    _, noisy = synth.noisy_clip(W, H, F, 3, 20.0, 4)
+    rng = np.random.default_rng(12)
+    ff = rng.uniform(-3.4, 3.4, (F - 1, 2, H, W)).astype(np.float32)
+    bf = rng.uniform(-3.4, 3.4, (F - 1, 2, H, W)).astype(np.float32)
+    p1, p2 = V.initialize_parameters_1(20.0), V.initialize_parameters_2(20.0)
+    nz = noisy.copy()
+    gb = np.zeros_like(nz)
+    gf = np.zeros_like(nz)
+    V.run_vbm3d(20.0, nz, ff, bf, gb, gf, p1, p2, True, 1, ctx=gpu_ctx)
+    ob, of, onz = oracle.run_vbm3d(20.0, noisy, p1, p2, True, 1, fflow=ff, bflow=bf)
+    assert_bits_equal(gb, ob, "basic")
+    assert_bits_equal(gf, of, "final")
+    assert_bits_equal(nz, onz, "noisy after the colour round trip")
+
+
 def test_run_chunked_ring(cfg1, oracle, gpu_ctx, monkeypatch):
     """A tiny estimate budget forces one-frame chunks and a ring of
     2 Nf + 1 frames; the result must not change."""
