@@ -18,8 +18,12 @@ profiles/traffic.json when one matches the workload.  `hbm` is the whole
 job's algorithmic HBM rate (20*C bytes per output pixel) against the HBM
 peak.  `cpu_baseline` is the CPU oracle (an op-order-faithful C++/OpenMP
 restatement of the reference path) timed on this host on a sample of the
-same workload; beside it, `fp64_error` is the GPU path's per-step error
-against the oracle's FP64 replay on a small clip.
+same workload; `accuracy` runs the GPU path on that same sample with the
+same reference grid and reports max |GPU - oracle| and the PSNR
+difference (the metric's "within 0.01 dB of reference"); `fp64_error` is
+the GPU path's per-step error against the oracle's FP64 replay on a small
+clip.  `config.batched` is the batched form's speed on the 64-clip mixed
+batch over the plain per-frame and per-clip GPU paths.
 """
 from __future__ import annotations
 
@@ -29,7 +33,11 @@ import os
 import sys
 import time
 
-import numpy as np
+# thread placement of the CPU baseline's OpenMP run, recorded with it
+os.environ.setdefault("OMP_PROC_BIND", "close")
+os.environ.setdefault("OMP_PLACES", "cores")
+
+import numpy as np  # noqa: E402
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
@@ -46,7 +54,9 @@ def pow2_floor(n: int) -> int:
 
 
 def cpu_baseline(noisy, sigma, frames, color=True):
-    """Time the oracle on the first `frames` frames (this host's cores)."""
+    """Time the oracle on the first `frames` frames (this host's cores).
+    Returns the JSON record and the oracle's outputs and counters, which
+    the accuracy leg compares the GPU path against."""
     from oracle import oracle as O
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     nb = pow2_floor(min(32, max(1, threads)))
@@ -54,13 +64,53 @@ def cpu_baseline(noisy, sigma, frames, color=True):
     F, C, H, W = sample.shape
     p1, p2 = O.params_1(sigma), O.params_2(sigma)
     t0 = time.perf_counter()
-    O.run_vbm3d(sigma, sample, p1, p2, color, nb, nb)
+    basic, final, _ = O.run_vbm3d(sigma, sample, p1, p2, color, nb, nb)
     dt = time.perf_counter() - t0
-    return {"value": round(W * H * F / dt / 1e6, 4), "unit": "Mpixel/s", "cores": nb,
-            "kind": "port",
-            "sample": f"first {F} of the frames, {W}x{H}x{C}, sigma {sigma:g}, both steps, "
-                      f"OpenMP over nb={nb} reference tiles (the reference's thread rule), "
-                      f"{dt:.1f} s"}
+    stats = {f"step{st}": O.last_stats(st) for st in (1, 2)}
+    rec = {"value": round(W * H * F / dt / 1e6, 4), "unit": "Mpixel/s", "cores": nb,
+           "kind": "port", "nproc": os.cpu_count(), "threads": nb,
+           "omp_proc_bind": os.environ.get("OMP_PROC_BIND"), "omp_places": os.environ.get("OMP_PLACES"),
+           "sample": f"first {F} of the frames, {W}x{H}x{C}, sigma {sigma:g}, both steps, "
+                     f"OpenMP over nb={nb} reference tiles on {nb} threads (the reference's thread "
+                     f"rule: the largest power of two <= min(32, OMP_NUM_THREADS)), {dt:.1f} s"}
+    return rec, {"nb": nb, "basic": basic, "final": final, "stats": stats, "sample": sample}
+
+
+def accuracy(api, synth, torch, ctx, clean, ref, sigma):
+    """The GPU path on the CPU baseline's sample, with the same reference
+    grid (nb), against the oracle's outputs: max |GPU - oracle| of the
+    basic and final estimates (0-255 scale), PSNR of both against the
+    clean clip and their difference, the candidate counts of both, and the
+    oracle's near-decision counters."""
+    dev = torch.device("cuda:0")
+    sample = ref["sample"]
+    F = sample.shape[0]
+    z = torch.from_numpy(sample).to(dev)
+    b, f = torch.empty_like(z), torch.empty_like(z)
+    p1, p2 = api.initialize_parameters_1(sigma), api.initialize_parameters_2(sigma)
+    api.profile(2, ctx)
+    api.run_device(sigma, z, b, f, p1, p2, True, ref["nb"], ctx=ctx)
+    torch.cuda.synchronize()
+    cands = [api.profile_read("block_match", ctx, st)[2] for st in (1, 2)]
+    api.profile(0, ctx)
+    gb, gf = b.cpu().numpy(), f.cpu().numpy()
+    cl = clean[:F]
+    p_gpu, p_orc = synth.psnr(cl, gf), synth.psnr(cl, ref["final"])
+    st = ref["stats"]
+    return {"sample": f"the CPU baseline's {F} frames, reference grid nb={ref['nb']}",
+            "basic_max_abs": float(np.abs(gb - ref["basic"]).max()),
+            "final_max_abs": float(np.abs(gf - ref["final"]).max()),
+            "bit_identical": bool(np.array_equal(gb.view(np.uint32), ref["basic"].view(np.uint32))
+                                  and np.array_equal(gf.view(np.uint32), ref["final"].view(np.uint32))),
+            "psnr_gpu_db": round(p_gpu, 4), "psnr_oracle_db": round(p_orc, 4),
+            "dpsnr_db": p_gpu - p_orc,
+            "candidates_gpu": [int(c) for c in cands],
+            "candidates_oracle": [st["step1"]["candidates"], st["step2"]["candidates"]],
+            "near_decisions": {k: {n: v[n] for n in ("refs", "near_tie_refs", "near_tau_refs",
+                                                     "near_threshold_coefs", "threshold_coefs")}
+                               for k, v in st.items()},
+            "near_rule": "a ranking gap, |distance - tau| or ||coefficient| - threshold| within 1e-5 "
+                         "(relative) in the oracle's FP32 run"}
 
 
 def fp64_error(api, synth):
@@ -107,6 +157,8 @@ def main():
                     help="instead: the 64-clip mixed batch, batched form against the per-clip path")
     ap.add_argument("--clips", type=int, default=64)
     ap.add_argument("--no-tune", action="store_true", help="batched: default launch configuration")
+    ap.add_argument("--no-batched", action="store_true",
+                    help="skip the batched-form ratio in the default line (config.batched)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -133,13 +185,28 @@ def main():
         res["config"]["sigma40"]["bm_roofline_frac"] = s40["roofline"]["frac"]
     if not args.no_cpu:
         print(f"[bench] cpu baseline on {args.cpu_frames} frames", file=sys.stderr, flush=True)
-        _, noisy = synth.noisy_clip(W, H, min(args.cpu_frames, F), C, sigma, args.seed, vmax=args.vmax)
-        res["cpu_baseline"] = cpu_baseline(noisy, sigma, min(args.cpu_frames, F))
+        clean, noisy = synth.noisy_clip(W, H, min(args.cpu_frames, F), C, sigma, args.seed, vmax=args.vmax)
+        res["cpu_baseline"], ref = cpu_baseline(noisy, sigma, min(args.cpu_frames, F))
+        res["accuracy"] = accuracy(api, synth, torch, ctx, clean, ref, sigma)
         res["fp64_error"] = fp64_error(api, synth)
+        if ref["nb"] != args.nb:
+            # the headline clip again with the CPU run's reference grid
+            print(f"[bench] GPU with the CPU's grid nb={ref['nb']}", file=sys.stderr, flush=True)
+            same = measure(api, synth, torch, ctx, stream, dev, args, sigma, nb=ref["nb"], light=True)
+            res["config"]["gpu_at_cpu_nb"] = {"nb": ref["nb"], "value": same["value"], "unit": "Mpixel/s",
+                                              "ms_per_step": same["ms_per_step"]}
+    if not args.no_batched:
+        print("[bench] batched form on the 64-clip mixed batch", file=sys.stderr, flush=True)
+        bargs = argparse.Namespace(**vars(args))
+        bargs.steps = 2
+        b = measure_batched(api, synth, torch, bargs, tune=False)
+        res["config"]["batched"] = {k: b["config"][k] for k in ("workload", "per_frame_ms", "per_clip_ms",
+                                                                "batched_ms", "over_per_clip", "bit_identical")}
+        res["config"]["batched"]["over_per_frame"] = b["value"]
     print(json.dumps(res), flush=True)
 
 
-def measure_batched(api, synth, torch, args):
+def measure_batched(api, synth, torch, args, tune=None):
     """64 clips (seeds 100..163, synth.batch_spec): the plain per-frame GPU
     path (vbm3d_run_device per clip, one launch per kernel and frame) and
     the per-clip path (one launch per kernel and step for the whole clip)
@@ -191,7 +258,8 @@ def measure_batched(api, synth, torch, args):
     torch.cuda.synchronize()
     t_clip = (time.perf_counter() - t0) / args.steps
     print(f"[bench] per-clip path {t_clip * 1e3:.1f} ms", file=sys.stderr, flush=True)
-    tuned = api.autotune_batch(items, ctx=ctx) if not args.no_tune else None
+    tune = (not args.no_tune) if tune is None else tune
+    tuned = api.autotune_batch(items, ctx=ctx) if tune else None
     print(f"[bench] autotuned {tuned}", file=sys.stderr, flush=True)
     batched()  # plans and captures the graph
     torch.cuda.synchronize()
@@ -240,8 +308,11 @@ def load_traffic(workload: str, kernel: str):
     return ent["fetch_bytes_per_launch"] + ent["write_bytes_per_launch"]
 
 
-def measure(api, synth, torch, ctx, stream, dev, args, sigma):
+def measure(api, synth, torch, ctx, stream, dev, args, sigma, nb=None, light=False):
+    """Times `args.steps` full runs of the clip at `sigma` on reference grid
+    `nb` (default args.nb).  light: the timing alone (no host-buffer run)."""
     W, H, F, C = args.width, args.height, args.frames, args.channels
+    nb = args.nb if nb is None else nb
     print(f"[bench] generating {F}x{C}x{H}x{W} sigma={sigma:g}", file=sys.stderr, flush=True)
     clean, noisy = synth.noisy_clip(W, H, F, C, sigma, args.seed, vmax=args.vmax)
     pristine = torch.from_numpy(noisy).to(dev)
@@ -253,7 +324,7 @@ def measure(api, synth, torch, ctx, stream, dev, args, sigma):
 
     def one_step():
         work.copy_(pristine)
-        api.run_device(sigma, work, basic, final, p1, p2, True, args.nb, stream=stream, ctx=ctx)
+        api.run_device(sigma, work, basic, final, p1, p2, True, nb, stream=stream, ctx=ctx)
 
     # the warmup (or an extra untimed step) counts the work per kernel; the
     # timed steps only bracket launches with events
@@ -295,13 +366,13 @@ def measure(api, synth, torch, ctx, stream, dev, args, sigma):
     # basic estimate and the result crossing PCIe; reported beside the
     # headline, never as it (the second of two calls, so buffers exist)
     host_ms = None
-    if sigma == args.sigma:
+    if sigma == args.sigma and not light:
         h_basic = np.empty_like(noisy)
         h_final = np.empty_like(noisy)
         for i in range(2):
             h_noisy = noisy.copy()
             t1 = time.perf_counter()
-            api.run_vbm3d(sigma, h_noisy, None, None, h_basic, h_final, p1, p2, True, args.nb, ctx=ctx)
+            api.run_vbm3d(sigma, h_noisy, None, None, h_basic, h_final, p1, p2, True, nb, ctx=ctx)
             host_ms = (time.perf_counter() - t1) * 1e3
         print(f"[bench] host-buffer run {host_ms:.1f} ms", file=sys.stderr, flush=True)
         if not np.array_equal(h_final, out):
@@ -310,7 +381,7 @@ def measure(api, synth, torch, ctx, stream, dev, args, sigma):
     psnr_noisy = synth.psnr(clean, noisy)
     mpix = W * H * F / dt / 1e6
     workload = (f"{F} {'RGB' if C == 3 else 'gray'} frames {W}x{H}, sigma={sigma:g}, both steps, "
-                f"default parameters, reference grid nb={args.nb}")
+                f"default parameters, reference grid nb={nb}")
     kname = kernel_name(prm, C)
     hbm_gbs = 20.0 * C * W * H * F / dt / 1e9
     return {
@@ -327,12 +398,14 @@ def measure(api, synth, torch, ctx, stream, dev, args, sigma):
                    "host_buffers": None if host_ms is None else {
                        "entry": "vbm3d_run (pageable host buffers, PCIe included)",
                        "ms": round(host_ms, 2), "mpix_s": round(W * H * F / host_ms / 1e3, 3)}},
-        "roofline": {"bound": "mfma", "kernel": f"{kname} (block matching, step {dom})",
+        "roofline": {"bound": "fp32-valu", "kernel": f"{kname} (block matching, step {dom})",
                      "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
                      "traffic": load_traffic(workload, kname),
-                     "peak_note": "FP32 vector peak (the MFMA FP32 figure is the same 157.3); "
-                                  "the kernel runs on VALU without FMA contraction or packed FP32",
+                     "peak_note": "MI355X FP32 vector peak (counts FMA as 2 and packed FP32); the kernel "
+                                  "issues no MFMA and may use neither FMA nor packed FP32 without changing "
+                                  "the reference's rounding, so 39.3 T non-packed operations/s is its "
+                                  "practical ceiling",
                      "avg_launch_ms": round(avg_ms, 3),
                      "flops_per_launch": flops_launch,
                      "candidates_per_step": cands[dom - 1]},

@@ -74,6 +74,8 @@ def lib():
         L.orc_bior_inv.argtypes = [fp]
         L.orc_dct_fwd.argtypes = [C.c_uint, fp, fp]
         L.orc_dct_inv.argtypes = [C.c_uint, fp]
+        L.orc_temporal_fwd.argtypes = [C.c_uint, fp]
+        L.orc_temporal_inv.argtypes = [C.c_uint, fp]
         L.orc_haar_fwd.argtypes = [fp, C.c_uint]
         L.orc_haar_inv.argtypes = [fp, C.c_uint]
         L.orc_hadamard.argtypes = [fp, C.c_uint]
@@ -84,6 +86,7 @@ def lib():
         L.orc_partial_sort.argtypes = [fp, up, C.c_uint, C.c_uint]
         L.orc_psnr.argtypes = [fp, fp, C.c_size_t, dp, dp]
         L.orc_last_error.restype = C.c_char_p
+        L.orc_last_stats.argtypes = [C.c_int, C.POINTER(C.c_longlong)]
         _lib = L
     return _lib
 
@@ -178,6 +181,17 @@ def step(stepno, sigma, noisy, basic, prm, color_space=True, nb=1, nthreads=0):
     return num, den
 
 
+STAT_NAMES = ("refs", "candidates", "members", "near_tie_refs", "near_tau_refs",
+              "near_threshold_coefs", "threshold_coefs")
+
+
+def last_stats(stepno):
+    """Counters of the last step / run for step 1 or 2 (orc_last_stats)."""
+    out = (C.c_longlong * 7)()
+    lib().orc_last_stats(stepno, out)
+    return dict(zip(STAT_NAMES, list(out)))
+
+
 def step_fp64(stepno, sigma, noisy, basic, prm, color_space=True):
     prm = as_orc(prm)
     nz = np.ascontiguousarray(noisy, np.float32)
@@ -230,6 +244,18 @@ def dct_fwd(patch):
 def dct_inv(coef, k):
     x = np.array(coef, np.float32, copy=True).reshape(-1)
     lib().orc_dct_inv(k, _f(x))
+    return x
+
+
+def temporal_fwd(v):
+    x = np.array(v, np.float32, copy=True)
+    lib().orc_temporal_fwd(len(x), _f(x))
+    return x
+
+
+def temporal_inv(v):
+    x = np.array(v, np.float32, copy=True)
+    lib().orc_temporal_inv(len(x), _f(x))
     return x
 
 

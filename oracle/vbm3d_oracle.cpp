@@ -45,6 +45,7 @@
 namespace {
 
 thread_local std::string g_err;
+long long g_stats[3][7];  // last orc_step's counters, per step (see StepIn::stats)
 
 const unsigned kDCT = 0, kBIOR = 1, kHADAMARD = 2;
 
@@ -73,6 +74,26 @@ inline float flow_at(const float* fl, const Geo& g, unsigned x, unsigned y,
                      unsigned t, unsigned c) {
     if (!fl) return 0.f;
     return fl[(size_t)t * g.wh * 2 + (size_t)c * g.wh + (size_t)y * g.W + x];
+}
+
+// patch_trajectory (src/vbm3d.cpp:378-425, FLOAT_TRAJECTORIES): the
+// top-left corner of each of the kt slices of the patch at (px, py, pt).
+// With mc, the next corner is the running float sum of the forward flow
+// read at the current corner plus k/2 (both components before either
+// moves), rounded half away from zero and clamped to the frame.
+inline void trajectory(unsigned px, unsigned py, unsigned pt, unsigned k, unsigned kt, bool mc,
+                       const float* ff, const Geo& g, unsigned* tx, unsigned* ty) {
+    float fpx = (float)px, fpy = (float)py;
+    for (unsigned ht = 0; ht < kt; ht++) {
+        tx[ht] = px;
+        ty[ht] = py;
+        if (mc && ht + 1 < kt) {
+            fpx += flow_at(ff, g, px + k / 2, py + k / 2, pt + ht, 0);
+            fpy += flow_at(ff, g, px + k / 2, py + k / 2, pt + ht, 1);
+            px = (unsigned)std::min((int)g.W - (int)k, std::max(0, (int)std::round(fpx)));
+            py = (unsigned)std::min((int)g.H - (int)k, std::max(0, (int)std::round(fpy)));
+        }
+    }
 }
 
 // ---------------------------------------------------------------------
@@ -303,11 +324,19 @@ void dct_norm(unsigned k, float* nrm, float* inv) {
 }
 
 struct Tables {
-    unsigned k;
+    unsigned k, kt;
     float biorF[64], biorI[64];          // float copies, row-major [j][s] / [s][j]
     float dctF[16 * 16], dctI[16 * 16];  // [v][q] REDFT10 and [q][v] REDFT01
     float nrm[256], inv[256], kaiser[256];
-    explicit Tables(unsigned kk) : k(kk) {
+    float tF[64], tI[64];  // temporal REDFT10 [u][t] and REDFT01 [t][u], length kt
+    float tnorm;           // 1/sqrt(2 kt) rounded to float (src/vbm3d.cpp:1876)
+    explicit Tables(unsigned kk, unsigned ktt = 1) : k(kk), kt(ktt) {
+        for (unsigned u = 0; u < kt; u++)
+            for (unsigned t = 0; t < kt; t++) {
+                tF[u * kt + t] = (float)(2.0 * std::cos(M_PI * (2.0 * t + 1.0) * u / (2.0 * kt)));
+                tI[t * kt + u] = u == 0 ? 1.0f : (float)(2.0 * std::cos(M_PI * u * (2.0 * t + 1.0) / (2.0 * kt)));
+            }
+        tnorm = (float)(1. / std::sqrt(2.0 * kt));
         for (int a = 0; a < 8; a++)
             for (int b = 0; b < 8; b++) {
                 biorF[a * 8 + b] = (float)kBiorF[a][b];
@@ -334,7 +363,8 @@ struct TabT {
     T biorF[64], biorI[64], dctF[256], dctI[256], nrm[256], inv[256];
     T kaiser[256];
     T dct_out;  // 1/(2k), src/vbm3d.cpp:1707
-    unsigned k;
+    T tF[64], tI[64], tnorm;  // temporal DCT (kt > 1)
+    unsigned k, kt;
 };
 
 TabT<float> make_tab_f(const Tables& t) {
@@ -348,6 +378,10 @@ TabT<float> make_tab_f(const Tables& t) {
     std::memcpy(o.inv, t.inv, sizeof o.inv);
     std::memcpy(o.kaiser, t.kaiser, sizeof o.kaiser);
     o.dct_out = 1.0f / (float)(t.k * 2);
+    o.kt = t.kt;
+    std::memcpy(o.tF, t.tF, sizeof o.tF);
+    std::memcpy(o.tI, t.tI, sizeof o.tI);
+    o.tnorm = t.tnorm;
     return o;
 }
 
@@ -374,7 +408,47 @@ TabT<double> make_tab_d(const Tables& t) {
         }
     for (unsigned a = 0; a < k * k; a++) o.kaiser[a] = t.kaiser[a];
     o.dct_out = 1.0 / (2.0 * k);
+    const unsigned kt = t.kt;
+    o.kt = kt;
+    for (unsigned u = 0; u < kt; u++)
+        for (unsigned tt = 0; tt < kt; tt++) {
+            o.tF[u * kt + tt] = 2.0 * std::cos(M_PI * (2.0 * tt + 1.0) * u / (2.0 * kt));
+            o.tI[tt * kt + u] = u == 0 ? 1.0 : 2.0 * std::cos(M_PI * u * (2.0 * tt + 1.0) / (2.0 * kt));
+        }
+    o.tnorm = 1.0 / std::sqrt(2.0 * kt);
     return o;
+}
+
+// temporal_transform (src/vbm3d.cpp:1866-1910) on one coefficient's kt
+// values x[t] (stride sx): REDFT10 as sequential sums over t with the
+// closed-form matrix, times 1/sqrt(2 kt) (a float product), then the DC
+// times SQRT2_INV in double, rounded once to float.
+template <class T>
+void temporal_fwd(T* x, unsigned sx, const TabT<T>& tb) {
+    T y[8];
+    const unsigned kt = tb.kt;
+    for (unsigned u = 0; u < kt; u++) {
+        T m = 0;
+        for (unsigned t = 0; t < kt; t++) m += x[t * sx] * tb.tF[u * kt + t];
+        y[u] = m;
+    }
+    for (unsigned u = 0; u < kt; u++)
+        x[u * sx] = (T)((double)(y[u] * tb.tnorm) * (u == 0 ? kSqrt2Inv : 1.0));
+}
+
+// temporal_inv_transform (src/vbm3d.cpp:1912-1956): the DC times SQRT2 in
+// double (rounded to float), REDFT01 as sequential sums over u, times
+// 1/sqrt(2 kt).
+template <class T>
+void temporal_inv(T* x, unsigned sx, const TabT<T>& tb) {
+    T z[8];
+    const unsigned kt = tb.kt;
+    for (unsigned u = 0; u < kt; u++) z[u] = (T)((double)x[u * sx] * (u == 0 ? kSqrt2 : 1.0));
+    for (unsigned t = 0; t < kt; t++) {
+        T m = 0;
+        for (unsigned u = 0; u < kt; u++) m += z[u] * tb.tI[t * kt + u];
+        x[t * sx] = m * tb.tnorm;
+    }
 }
 
 // ---------------------------------------------------------------------
@@ -561,6 +635,17 @@ void hadamard(T* v, unsigned n) {
     hadamard(v + h, h);
 }
 
+// Relative distance below which two ranked values, a distance and tau, or
+// a coefficient and the hard threshold count as a near tie: a few hundred
+// FP32 ulps, the size of the rounding differences another operation order
+// (FFTW's DCT, a -ffast-math build) could introduce.
+constexpr double kNearRel = 1e-5;
+inline bool near_rel(double a, double b) {
+    return std::fabs(a - b) <= kNearRel * std::max(1.0, std::max(std::fabs(a), std::fabs(b)));
+}
+
+// A ranking decision near a tie: among the `keep` smallest of `v` (sorted
+// copy), or between the last kept and the first dropped value.
 // ---------------------------------------------------------------------
 // Shrinkage along the group.  Group layout g[n + kk*nsx + c*k2*nsx]
 // (src/vbm3d.cpp:569, 810-813).  `keep` records (FP32) or replays (FP64)
@@ -568,7 +653,8 @@ void hadamard(T* v, unsigned n) {
 // ---------------------------------------------------------------------
 template <class T>
 void ht_filter(T* g, unsigned nsx, unsigned k2, unsigned C, const float* sig,
-               float lambda, unsigned T3D, T* w, uint8_t* keep, bool replay) {
+               float lambda, unsigned T3D, T* w, uint8_t* keep, bool replay,
+               long long* near_thr = nullptr) {
     const unsigned per_c = k2 * nsx;
     const bool had = T3D == kHADAMARD;
     for (unsigned a = 0; a < k2 * C; a++) {
@@ -588,6 +674,7 @@ void ht_filter(T* g, unsigned nsx, unsigned k2, unsigned C, const float* sig,
             else {
                 kp = (k == 0 || std::fabs((float)v) > thr);
                 keep[c * per_c + k] = kp;
+                if (near_thr && k != 0 && near_rel(std::fabs((double)v), (double)thr)) (*near_thr)++;
             }
             if (kp) n_keep++;
             else v = 0;
@@ -674,6 +761,14 @@ struct Seen {
     }
 };
 
+inline bool ranking_near_tie(std::vector<float> v, size_t keep) {
+    std::sort(v.begin(), v.end());
+    const size_t upto = std::min(keep + 1, v.size());
+    for (size_t j = 1; j < upto; j++)
+        if (near_rel(v[j - 1], v[j])) return true;
+    return false;
+}
+
 struct BM {
     const float* vid;
     Geo g;
@@ -681,24 +776,34 @@ struct BM {
     orc_params pr;
     Seen seen;
     long long n_eval = 0;
+    long long near_tie = 0;  // references with a ranking decision near a tie
+    long long near_tau = 0;  // references with a group candidate near tau
+    bool tie_now = false;
     BM(const float* v, const Geo& gg, const float* f, const float* b, const orc_params& p)
         : vid(v), g(gg), ff(f), bf(b), pr(p),
           seen(p.Ns * p.Ns + 2 * p.Nf * p.Nb * p.Npr * p.Npr + 1) {}
 
-    // patchDistance with kt = 1 (src/vbm3d.cpp:930-1011).
+    // patchDistance (src/vbm3d.cpp:930-1011): both patches follow their
+    // trajectories over the kt slices (identity unless mc); one sequential
+    // float sum in (slice, channel, row, column) order.
     float dist(unsigned r, unsigned q) const {
         unsigned rx, ry, rt, qx, qy, qt;
         g.coords(r, rx, ry, rt);
         g.coords(q, qx, qy, qt);
-        const unsigned k = pr.k;
+        const unsigned k = pr.k, kt = pr.kt;
+        unsigned ax[16], ay[16], bx[16], by[16];
+        trajectory(rx, ry, rt, k, kt, pr.mc != 0, ff, g, ax, ay);
+        trajectory(qx, qy, qt, k, kt, pr.mc != 0, ff, g, bx, by);
         float d = 0.f, dif;
-        for (unsigned c = 0; c < g.C; ++c)
-            for (unsigned hy = 0; hy < k; hy++)
-                for (unsigned hx = 0; hx < k; hx++) {
-                    dif = vid[g.index(rx + hx, ry + hy, rt, c)] - vid[g.index(qx + hx, qy + hy, qt, c)];
-                    d += dif * dif;
-                }
-        return d / (float)(k * k * pr.kt * g.C);
+        for (unsigned ht = 0; ht < kt; ht++)
+            for (unsigned c = 0; c < g.C; ++c)
+                for (unsigned hy = 0; hy < k; hy++)
+                    for (unsigned hx = 0; hx < k; hx++) {
+                        dif = vid[g.index(ax[ht] + hx, ay[ht] + hy, rt + ht, c)] -
+                              vid[g.index(bx[ht] + hx, by[ht] + hy, qt + ht, c)];
+                        d += dif * dif;
+                    }
+        return d / (float)(k * k * kt * g.C);
     }
 
     // localSearch (src/vbm3d.cpp:1013-1080).
@@ -721,8 +826,15 @@ struct BM {
                 lst.push_back(Cand(d, cur));
             }
         const int nb = std::min((int)pr.Nb, (int)lst.size());
+        tie_now = tie_now || ranking_near_tie(dists(lst), nb);
         std::partial_sort(lst.begin(), lst.begin() + nb, lst.end(), cmp_first);
         for (int i = 0; i < nb; i++) out.push_back(lst[i]);
+    }
+
+    static std::vector<float> dists(const std::vector<Cand>& v) {
+        std::vector<float> d(v.size());
+        for (size_t i = 0; i < v.size(); i++) d[i] = v[i].first;
+        return d;
     }
 
     // Flow projection of a match into the next (dir=+1) or previous frame
@@ -743,6 +855,7 @@ struct BM {
     // computeSimilarPatches (src/vbm3d.cpp:1082-1197).  Returns nSx.
     unsigned run(unsigned pidx, float* od, unsigned* oi, unsigned* nsim_out) {
         seen.reset();
+        tie_now = false;
         std::vector<Cand> best;
         best.reserve(pr.Nb * (2 * pr.Nf + 1));
         local(pidx, pidx, pr.Ns, best);
@@ -756,6 +869,7 @@ struct BM {
             fb.clear();
             for (unsigned m = 0; m < pr.Nb; m++) local(project(post[m], +1), pidx, pr.Npr, fb);
             const int nb = std::min((int)pr.Nb, (int)fb.size());
+            tie_now = tie_now || ranking_near_tie(dists(fb), nb);
             std::partial_sort(fb.begin(), fb.begin() + nb, fb.end(), cmp_first);
             for (int i = 0; i < nb; i++) { post[i] = fb[i].second; best.push_back(fb[i]); }
         }
@@ -764,12 +878,18 @@ struct BM {
             fb.clear();
             for (unsigned m = 0; m < pr.Nb; m++) local(project(pre[m], -1), pidx, pr.Npr, fb);
             const int nb = std::min((int)pr.Nb, (int)fb.size());
+            tie_now = tie_now || ranking_near_tie(dists(fb), nb);
             std::partial_sort(fb.begin(), fb.begin() + nb, fb.end(), cmp_first);
             for (int i = 0; i < nb; i++) { pre[i] = fb[i].second; best.push_back(fb[i]); }
         }
         const unsigned nsim = std::min(pr.N, (unsigned)best.size());
+        tie_now = tie_now || ranking_near_tie(dists(best), nsim);
+        near_tie += tie_now;
         std::partial_sort(best.begin(), best.begin() + nsim, best.end(), cmp_first);
         for (unsigned n = 0; n < nsim; n++) { od[n] = best[n].first; oi[n] = best[n].second; }
+        bool nt = false;
+        for (unsigned n = 1; n < nsim; n++) nt = nt || near_rel(od[n], pr.tau);
+        near_tau += nt;
         unsigned it = nsim - 1;
         while (od[it] > pr.tau && it > 0) it--;
         if (nsim_out) *nsim_out = nsim;
@@ -784,9 +904,10 @@ bool check_params(const orc_params& p, const Geo& g, int step) {
     char buf[256];
     auto fail = [&](const char* m) { std::snprintf(buf, sizeof buf, "step %d: %s", step, m); g_err = buf; return false; };
     if (p.k == 0) return fail("k must be > 0");
-    if (p.kt != 1) return fail("only kt == 1 is supported");
+    if (p.kt == 0 || p.kt > 8) return fail("kt must be in [1, 8]");
+    if (p.kt > g.F) return fail("fewer frames than kt");
     if (p.k > 16) return fail("k > 16 is not supported");
-    if (p.N == 0 || p.N > 32 || !is_pow2(p.N)) return fail("N must be a power of two in [1, 32]");
+    if (p.N == 0 || p.N > 32) return fail("N must be in [1, 32]");
     if (p.Nb == 0 || p.Nb > 16) return fail("Nb must be in [1, 16]");
     if (p.Ns == 0 || p.Npr == 0 || p.Ns > 31 || p.Npr > 31) return fail("Ns and Npr must be in [1, 31]");
     if (p.p == 0) return fail("p must be > 0");
@@ -803,14 +924,20 @@ bool check_params(const orc_params& p, const Geo& g, int step) {
     return true;
 }
 
-// Largest distance of a group member from its reference patch.
+// Largest distance of a group member from its reference patch (sizes the
+// crop's aggregation box), in 64 bits and capped at the frame size: a
+// projection is clamped to the frame, so no member lies farther.
 unsigned reach(const orc_params& p, const float* ff, const float* bf, const Geo& g) {
     float mf = 0.f;
     const size_t n = (size_t)(g.F > 1 ? g.F - 1 : 0) * g.wh * 2;
     for (const float* fl : {ff, bf})
         if (fl)
-            for (size_t i = 0; i < n; i++) mf = std::max(mf, std::fabs(std::round(fl[i])));
-    return (p.Ns - 1) / 2 + p.Nf * ((p.Npr - 1) / 2 + (unsigned)mf);
+            for (size_t i = 0; i < n; i++) mf = std::max(mf, std::fmin(std::fabs(std::round(fl[i])), 65535.f));
+    unsigned long long r = (p.Ns - 1) / 2 + (unsigned long long)p.Nf * ((p.Npr - 1) / 2 + (unsigned long long)mf);
+    // a motion-compensated trajectory drifts by at most the rounded sum of
+    // kt - 1 flow steps
+    if (p.mc && p.kt > 1) r += (unsigned long long)(p.kt - 1) * ((unsigned long long)mf + 1);
+    return (unsigned)std::min<unsigned long long>(r, std::max(g.W, g.H));
 }
 
 // ---------------------------------------------------------------------
@@ -831,7 +958,11 @@ struct StepIn {
     Geo g;
     orc_params pr;
     float sig[3];
-    long long stats[4];
+    // references, candidates evaluated, group members, references with a
+    // ranking decision near a tie, references with a group candidate near
+    // tau, hard-threshold coefficients near the threshold, coefficients
+    // tested against it (the 3D DC is exempt)
+    long long stats[7];
 };
 
 template <class T>
@@ -841,8 +972,9 @@ void run_crop(StepIn& in, const std::vector<unsigned>& rows,
               std::vector<uint32_t>* grp_log, size_t* grp_pos) {
     const Geo& g = in.g;
     const orc_params& pr = in.pr;
-    const unsigned k = pr.k, k2 = k * k, C = g.C, N = pr.N;
-    Tables tables(k);
+    const unsigned k = pr.k, k2 = k * k, kt = pr.kt, k2t = k2 * kt, C = g.C, N = pr.N;
+    const bool temporal = kt > 1 && pr.T_2D != kBIOR;  // src/vbm3d.cpp:573, 818
+    Tables tables(k, kt);
     TabT<float> tf = make_tab_f(tables);
     TabT<double> td = make_tab_d(tables);
     const TabT<T>& tb = *reinterpret_cast<const TabT<T>*>(
@@ -861,9 +993,10 @@ void run_crop(StepIn& in, const std::vector<unsigned>& rows,
     BM bm(mvid, g, in.ff, in.bf, pr);
     std::vector<float> dist(N);
     std::vector<unsigned> idx(N);
-    std::vector<T> tz((size_t)C * N * k2), te((size_t)C * N * k2), gz, ge, patch(k2);
-    std::vector<uint8_t> keep((size_t)C * N * k2);
+    std::vector<T> tz((size_t)C * N * k2t), te((size_t)C * N * k2t), gz, ge, patch(k2);
+    std::vector<uint8_t> keep((size_t)C * N * k2t);
     T w[3];
+    unsigned trx[16], tr_y[16];
 
     for (unsigned t = 0; t + pr.kt <= g.F; ++t)
         for (unsigned yr : rows)
@@ -882,70 +1015,98 @@ void run_crop(StepIn& in, const std::vector<unsigned>& rows,
                 }
                 in.stats[0]++;
                 in.stats[2] += nsx;
-                // 2D transforms of the group members (dct_2d_process
-                // :1218-1304, bior_2d_process :1325-1386), table[c][n][kk]
+                // 2D transforms of every slice of the group members along
+                // their trajectories (dct_2d_process :1218-1304,
+                // bior_2d_process :1325-1386), table[c][n][slice][kk]
                 for (unsigned c = 0; c < C; c++)
                     for (unsigned n = 0; n < nsx; n++) {
                         unsigned px, py, pt;
                         g.coords(idx[n], px, py, pt);
+                        trajectory(px, py, pt, k, kt, pr.mc != 0, in.ff, g, trx, tr_y);
                         for (int which = 0; which < (in.step == 2 ? 2 : 1); which++) {
                             const float* src = which == 0 ? in.noisy : in.basic;
-                            for (unsigned p = 0; p < k; p++)
-                                for (unsigned q = 0; q < k; q++)
-                                    patch[p * k + q] = (T)src[g.index(px + q, py + p, pt, c)];
-                            T* dst = (which == 0 ? tz.data() : te.data()) + (size_t)(c * N + n) * k2;
-                            if (pr.T_2D == kBIOR) bior_fwd(patch.data(), dst, tb);
-                            else dct_fwd(patch.data(), dst, tb);
+                            for (unsigned ht = 0; ht < kt; ht++) {
+                                for (unsigned p = 0; p < k; p++)
+                                    for (unsigned q = 0; q < k; q++)
+                                        patch[p * k + q] = (T)src[g.index(trx[ht] + q, tr_y[ht] + p, pt + ht, c)];
+                                T* dst = (which == 0 ? tz.data() : te.data()) + (size_t)(c * N + n) * k2t + ht * k2;
+                                if (pr.T_2D == kBIOR) bior_fwd(patch.data(), dst, tb);
+                                else dct_fwd(patch.data(), dst, tb);
+                            }
                         }
                     }
-                // group layout [c][kk][n] (:565-570, :803-815)
-                gz.assign((size_t)C * k2 * nsx, (T)0);
+                // group layout [c][kk][n], kk = slice * k2 + pixel
+                // (:565-570, :803-815)
+                gz.assign((size_t)C * k2t * nsx, (T)0);
                 ge.assign(gz.size(), (T)0);
                 for (unsigned c = 0; c < C; c++)
                     for (unsigned n = 0; n < nsx; n++)
-                        for (unsigned kk = 0; kk < k2; kk++) {
-                            gz[n + kk * nsx + c * k2 * nsx] = tz[(size_t)(c * N + n) * k2 + kk];
-                            if (in.step == 2) ge[n + kk * nsx + c * k2 * nsx] = te[(size_t)(c * N + n) * k2 + kk];
+                        for (unsigned kk = 0; kk < k2t; kk++) {
+                            gz[n + kk * nsx + c * k2t * nsx] = tz[(size_t)(c * N + n) * k2t + kk];
+                            if (in.step == 2) ge[n + kk * nsx + c * k2t * nsx] = te[(size_t)(c * N + n) * k2t + kk];
                         }
+                // transform along each coefficient's slices (:573-575, :816-822)
+                if (temporal)
+                    for (T* grp : {gz.data(), ge.data()}) {
+                        if (grp == ge.data() && in.step == 1) break;
+                        for (unsigned c = 0; c < C; c++)
+                            for (unsigned n = 0; n < nsx; n++)
+                                for (unsigned kk = 0; kk < k2; kk++)
+                                    temporal_fwd(grp + n + kk * nsx + c * k2t * nsx, k2 * nsx, tb);
+                    }
                 T* est;
                 if (in.step == 1) {
                     uint8_t* kp = keep.data();
-                    if (replay) { kp = keep_log->data() + *keep_pos; *keep_pos += (size_t)C * k2 * nsx; }
-                    ht_filter(gz.data(), nsx, k2, C, in.sig, pr.lambda3D, pr.T_3D, w, kp, replay);
-                    if (!replay && keep_log) keep_log->insert(keep_log->end(), kp, kp + (size_t)C * k2 * nsx);
+                    if (replay) { kp = keep_log->data() + *keep_pos; *keep_pos += (size_t)C * k2t * nsx; }
+                    ht_filter(gz.data(), nsx, k2t, C, in.sig, pr.lambda3D, pr.T_3D, w, kp, replay, &in.stats[5]);
+                    in.stats[6] += (long long)C * k2t * nsx - C;
+                    if (!replay && keep_log) keep_log->insert(keep_log->end(), kp, kp + (size_t)C * k2t * nsx);
                     est = gz.data();
                 } else {
-                    wiener_filter(gz.data(), ge.data(), nsx, k2, C, in.sig, pr.T_3D, w);
+                    wiener_filter(gz.data(), ge.data(), nsx, k2t, C, in.sig, pr.T_3D, w);
                     est = ge.data();
                 }
+                if (temporal)  // :589, :829-831
+                    for (unsigned c = 0; c < C; c++)
+                        for (unsigned n = 0; n < nsx; n++)
+                            for (unsigned kk = 0; kk < k2; kk++)
+                                temporal_inv(est + n + kk * nsx + c * k2t * nsx, k2 * nsx, tb);
                 // the group chunks in [c][n][kk] order (:592-595, :841-844),
-                // inverse 2D transform per chunk (:603-609, :852-858)
-                std::vector<T> chunk((size_t)C * nsx * k2);
+                // inverse 2D transform per slice (:603-609, :852-858)
+                std::vector<T> chunk((size_t)C * nsx * k2t);
                 for (unsigned c = 0; c < C; c++)
                     for (unsigned n = 0; n < nsx; n++)
-                        for (unsigned kk = 0; kk < k2; kk++)
-                            chunk[(size_t)(c * nsx + n) * k2 + kk] = est[n + kk * nsx + c * k2 * nsx];
-                for (unsigned a = 0; a < C * nsx; a++) {
+                        for (unsigned kk = 0; kk < k2t; kk++)
+                            chunk[(size_t)(c * nsx + n) * k2t + kk] = est[n + kk * nsx + c * k2t * nsx];
+                for (unsigned a = 0; a < C * nsx * kt; a++) {
                     if (pr.T_2D == kBIOR) bior_inv(chunk.data() + (size_t)a * k2, tb);
                     else dct_inv(chunk.data() + (size_t)a * k2, tb);
                 }
-                // aggregation (:612-650, :860-924)
+                // aggregation along the members' trajectories (:612-650,
+                // :860-924)
                 for (unsigned c = 0; c < C; c++)
                     for (unsigned n = 0; n < nsx; n++) {
                         unsigned px, py, pt;
                         g.coords(idx[n], px, py, pt);
-                        if (px < b.x0 || py < b.y0 || px + k > b.x0 + b.w || py + k > b.y0 + b.h)
-                            throw std::runtime_error("aggregation outside the crop box");
-                        const T* e = chunk.data() + (size_t)(c * nsx + n) * k2;
-                        for (unsigned p = 0; p < k; p++)
-                            for (unsigned q = 0; q < k; q++) {
-                                const size_t o = (((size_t)pt * C + c) * b.h + (py + p - b.y0)) * b.w + (px + q - b.x0);
-                                out.num[o] += tb.kaiser[p * k + q] * w[c] * e[p * k + q];
-                                out.den[o] += tb.kaiser[p * k + q] * w[c];
-                            }
+                        trajectory(px, py, pt, k, kt, pr.mc != 0, in.ff, g, trx, tr_y);
+                        for (unsigned ht = 0; ht < kt; ht++) {
+                            const unsigned sx = trx[ht], sy = tr_y[ht];
+                            if (sx < b.x0 || sy < b.y0 || sx + k > b.x0 + b.w || sy + k > b.y0 + b.h)
+                                throw std::runtime_error("aggregation outside the crop box");
+                            const T* e = chunk.data() + (size_t)(c * nsx + n) * k2t + ht * k2;
+                            for (unsigned p = 0; p < k; p++)
+                                for (unsigned q = 0; q < k; q++) {
+                                    const size_t o = (((size_t)(pt + ht) * C + c) * b.h + (sy + p - b.y0)) * b.w +
+                                                     (sx + q - b.x0);
+                                    out.num[o] += tb.kaiser[p * k + q] * w[c] * e[p * k + q];
+                                    out.den[o] += tb.kaiser[p * k + q] * w[c];
+                                }
+                        }
                     }
             }
     in.stats[1] += bm.n_eval;
+    in.stats[3] += bm.near_tie;
+    in.stats[4] += bm.near_tau;
 }
 
 // Full step: grid, crops, per-crop work (OpenMP over crops like
@@ -981,7 +1142,7 @@ int run_step(StepIn& in, int nb, int nthreads, T* num, T* den, bool replay,
     }
     if (!err.empty()) { g_err = err; return -1; }
     for (int c = 0; c < ncrop; c++)
-        for (int s = 0; s < 4; s++) in.stats[s] += ins[c].stats[s];
+        for (int s = 0; s < 7; s++) in.stats[s] += ins[c].stats[s];
     const unsigned C = g.C;
     for (unsigned t = 0; t < g.F; t++)
         for (unsigned ch = 0; ch < C; ch++)
@@ -1270,7 +1431,14 @@ int orc_step(int step, float sigma, const float* noisy, const float* basic,
              int nb, int nthreads, float* num, float* den) {
     StepIn in;
     if (!prepare(in, step, sigma, noisy, basic, fflow, bflow, W, H, F, C, prm, color_space)) return -1;
-    return run_step<float>(in, nb, nthreads, num, den, false, nullptr, nullptr);
+    const int rc = run_step<float>(in, nb, nthreads, num, den, false, nullptr, nullptr);
+    if (rc == 0 && (step == 1 || step == 2))
+        for (int s = 0; s < 7; s++) g_stats[step][s] = in.stats[s];
+    return rc;
+}
+
+void orc_last_stats(int step, long long* out) {
+    for (int s = 0; s < 7; s++) out[s] = (step == 1 || step == 2) ? g_stats[step][s] : 0;
 }
 
 int orc_step_fp64(int step, float sigma, const float* noisy, const float* basic,
@@ -1295,6 +1463,7 @@ int orc_step2_adjoint_fp64(float sigma, const float* noisy, const float* basic,
                            double* grad_noisy, double* final_out) {
     StepIn in;
     if (!basic) { g_err = "step 2 needs the basic estimate"; return -1; }
+    if (prm->kt != 1) { g_err = "the adjoint is restated for kt == 1"; return -1; }
     if (!prepare(in, 2, sigma, noisy, basic, fflow, bflow, W, H, F, C, prm, color_space)) return -1;
     Grid grid;
     if (!make_grid(grid, W, H, prm->k, prm->p, prm->n, nb)) { g_err = "invalid reference grid"; return -1; }
@@ -1380,6 +1549,16 @@ void orc_dct_inv(unsigned k, float* sig) {
     Tables t(k);
     TabT<float> tb = make_tab_f(t);
     dct_inv(sig, tb);
+}
+void orc_temporal_fwd(unsigned kt, float* v) {
+    Tables t(8, kt);
+    TabT<float> tb = make_tab_f(t);
+    temporal_fwd(v, 1, tb);
+}
+void orc_temporal_inv(unsigned kt, float* v) {
+    Tables t(8, kt);
+    TabT<float> tb = make_tab_f(t);
+    temporal_inv(v, 1, tb);
 }
 void orc_haar_fwd(float* v, unsigned n) { haar_fwd(v, n); }
 void orc_haar_inv(float* v, unsigned n) { haar_inv(v, n); }

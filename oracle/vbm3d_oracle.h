@@ -80,6 +80,14 @@ int orc_step(int step, float sigma, const float* noisy, const float* basic,
              unsigned F, unsigned C, const orc_params* prm, int color_space,
              int nb, int nthreads, float* num, float* den);
 
+/* Counters of the last orc_step (or orc_run_vbm3d) for step 1 or 2:
+ * out[0] reference patches, [1] candidates evaluated (patchDistance
+ * calls), [2] group members, [3] references with a block-matching
+ * ranking decision within 1e-5 (relative) of a tie, [4] references with a
+ * group candidate within 1e-5 of tau, [5] hard-threshold coefficients
+ * within 1e-5 of the threshold, [6] coefficients tested against it. */
+void orc_last_stats(int step, long long* out);
+
 /* FP64 re-evaluation of one step with the FP32 run's decisions frozen
  * (same groups, same hard-threshold keep/zero decisions).  Used to report
  * the FP32 path's per-step error.  Produces num/den in double. */
@@ -105,6 +113,10 @@ void orc_bior_fwd(const float* patch64, float* out64);
 void orc_bior_inv(float* sig64);
 void orc_dct_fwd(unsigned k, const float* patch, float* out);
 void orc_dct_inv(unsigned k, float* sig);
+/* temporal_transform / temporal_inv_transform (src/vbm3d.cpp:1866-1956)
+ * of one coefficient's kt (<= 8) values, in place. */
+void orc_temporal_fwd(unsigned kt, float* v);
+void orc_temporal_inv(unsigned kt, float* v);
 void orc_haar_fwd(float* v, unsigned n);
 void orc_haar_inv(float* v, unsigned n);
 void orc_hadamard(float* v, unsigned n);

@@ -197,6 +197,18 @@ def _video(a, name):
     return a
 
 
+def _flow(fl, name, F, H, W):
+    """A flow as the C ABI reads it: float32 [F'][2][H][W] with F' >= F - 1
+    (src/main.cpp:353-369), or None (zero flow).  A one-frame clip has no
+    predictive search, so its flows are never read."""
+    if fl is None or F <= 1:
+        return None
+    a = np.ascontiguousarray(fl, np.float32)
+    if a.ndim != 4 or a.shape[1:] != (2, H, W) or a.shape[0] < F - 1:
+        raise ValueError(f"{name} must have shape ({F - 1}, 2, {H}, {W}), got {a.shape}")
+    return a
+
+
 def run_vbm3d(sigma, vid_noisy, fflow, bflow, vid_basic, vid_denoised, prms_1, prms_2,
               color_space=True, ref_grid_nb=1, ctx: Context | None = None) -> int:
     """run_vbm3d (src/vbm3d.cpp:87-376).  vid_noisy is mutated by the
@@ -209,8 +221,8 @@ def run_vbm3d(sigma, vid_noisy, fflow, bflow, vid_basic, vid_denoised, prms_1, p
     F, Cc, H, W = vid_noisy.shape
     if vid_basic.shape != vid_noisy.shape or vid_denoised.shape != vid_noisy.shape:
         raise ValueError("vid_basic / vid_denoised must have vid_noisy's shape")
-    ff = None if fflow is None else np.ascontiguousarray(fflow, np.float32)
-    bf = None if bflow is None else np.ascontiguousarray(bflow, np.float32)
+    ff = _flow(fflow, "fflow", F, H, W)
+    bf = _flow(bflow, "bflow", F, H, W)
     ctx = ctx or default_context()
     _check(lib().vbm3d_run(ctx.h, float(sigma), _f(vid_noisy), _f(ff), _f(bf), W, H, F, Cc,
                            C.byref(prms_1), C.byref(prms_2), int(bool(color_space)), int(ref_grid_nb),
@@ -400,8 +412,8 @@ def block_match(vid, prm, ref_idx, fflow=None, bflow=None, ctx=None):
     dist = np.empty((n, N), np.float32)
     nsim = np.empty(n, np.uint32)
     nsx = np.empty(n, np.uint32)
-    ff = None if fflow is None else np.ascontiguousarray(fflow, np.float32)
-    bf = None if bflow is None else np.ascontiguousarray(bflow, np.float32)
+    ff = _flow(fflow, "fflow", F, H, W)
+    bf = _flow(bflow, "bflow", F, H, W)
     ctx = ctx or default_context()
     _check(lib().vbm3d_block_match(ctx.h, _f(vid), W, H, F, Cc, _f(ff), _f(bf), C.byref(prm), n,
                                    _u(ref_idx), _u(idx), _f(dist), _u(nsim), _u(nsx)))
@@ -414,8 +426,8 @@ def step(stepno, sigma, noisy, basic, prm, color_space=True, ref_grid_nb=1, fflo
     nz = np.ascontiguousarray(noisy, np.float32)
     F, Cc, H, W = nz.shape
     bs = None if basic is None else np.ascontiguousarray(basic, np.float32)
-    ff = None if fflow is None else np.ascontiguousarray(fflow, np.float32)
-    bf = None if bflow is None else np.ascontiguousarray(bflow, np.float32)
+    ff = _flow(fflow, "fflow", F, H, W)
+    bf = _flow(bflow, "bflow", F, H, W)
     num = np.empty_like(nz)
     den = np.empty_like(nz)
     ctx = ctx or default_context()

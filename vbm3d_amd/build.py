@@ -18,12 +18,15 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libvbm3d_hip.so")
-SOURCES = [os.path.join(CSRC, "kernels.hip"), os.path.join(CSRC, "capi.hip")]
+# one translation unit per kernel family, compiled in parallel
+SOURCES = [os.path.join(CSRC, f) for f in ("kernels_bm.hip", "kernels_filter.hip", "kernels_agg.hip",
+                                           "capi.hip")]
+OBJDIR = os.path.join(HERE, "_build")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("VBM3D_ARCH", "gfx950")
 
 FLAGS = [
-    f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",
+    f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",
     "-ffp-contract=off", "-fhip-fp32-correctly-rounded-divide-sqrt",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function", "-Werror=uninitialized",
@@ -48,8 +51,20 @@ def build(verbose: bool = False) -> str:
     """Compile libvbm3d_hip.so (HIP, gfx950), then the C++ host library
     libvbm3d_host.so (run_vbm3d over the C ABI) and the VBM3Ddenoising
     command line, both plain host C++ linked against libvbm3d_hip.so."""
+    os.makedirs(OBJDIR, exist_ok=True)
+    objs, procs = [], []
+    for src in SOURCES:
+        obj = os.path.join(OBJDIR, os.path.basename(src) + ".o")
+        cmd = [HIPCC, *FLAGS, "-c", "-o", obj, src]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        procs.append((cmd, subprocess.Popen(cmd)))
+        objs.append(obj)
+    for cmd, p in procs:
+        if p.wait() != 0:
+            raise subprocess.CalledProcessError(p.returncode, cmd)
     tmp = LIB + ".tmp"
-    _run([HIPCC, *FLAGS, "-o", tmp, *SOURCES], verbose)
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-o", tmp, *objs], verbose)
     os.replace(tmp, LIB)
     host_src = [os.path.join(HOST, "vbm3d_host.cpp"), os.path.join(HOST, "image_io.cpp")]
     _run([CXX, *CXXFLAGS, "-fPIC", "-shared", "-o", HOST_LIB + ".tmp", *host_src,

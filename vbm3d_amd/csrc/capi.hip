@@ -240,6 +240,17 @@ int check(const vbm3d_params& p, unsigned W, unsigned H, unsigned F, unsigned C,
     return VBM3D_OK;
 }
 
+// Largest displacement between a group member and its reference patch:
+// the reference-frame window's half width, plus per predictive frame the
+// window's half width and the largest rounded flow.  Evaluated in 64 bits
+// and capped at the frame size (a projection is clamped to the frame, so
+// no member lies farther), so that a huge or non-finite flow value cannot
+// wrap it.
+int member_reach(const vbm3d_params& p, unsigned maxflow, unsigned W, unsigned H) {
+    const unsigned long long r = (p.Ns - 1) / 2 + (unsigned long long)p.Nf * ((p.Npr - 1) / 2 + (unsigned long long)maxflow);
+    return (int)std::min<unsigned long long>(r, std::max(W, H));
+}
+
 }  // namespace
 
 struct ProfEvent {
@@ -352,6 +363,12 @@ int run_step(vbm3d_ctx* ctx, const StepIO& io, hipStream_t s) {
     const unsigned nrf = (unsigned)refxy.size();
     const unsigned K = p.k, K2 = K * K, C = io.C, N = p.N;
     const unsigned Fr = io.F - p.kt + 1;
+    Tabs tabs;
+    fill_tabs(tabs, K);
+    HIP_TRY(ensure(ctx->tabs, sizeof(Tabs)));
+    HIP_TRY(ensure(ctx->refxy, refxy.size() * 4));
+    HIP_TRY(ensure(ctx->meta, meta.size() * 4));
+    HIP_TRY(ensure(ctx->gpos, gpos.size() * 2));
     // chunk size from the estimate budget; ring holds S + 2 Nf frames
     const size_t per_frame = (size_t)nrf * C * N * K2 * sizeof(float);
     const size_t budget = ring_budget(ctx, ctx->est.cap + ctx->idx.cap + ctx->nsx.cap + ctx->wgt.cap);
@@ -364,16 +381,10 @@ int run_step(vbm3d_ctx* ctx, const StepIO& io, hipStream_t s) {
     const size_t slots = (size_t)ring * nrf;
     if ((slots * kMaxN) >> 32) return fail(VBM3D_ERR_UNSUPPORTED, "too many reference patches in flight");
 
-    Tabs tabs;
-    fill_tabs(tabs, K);
     HIP_TRY(ensure(ctx->idx, slots * N * 4));
     HIP_TRY(ensure(ctx->nsx, slots * 4));
     HIP_TRY(ensure(ctx->est, slots * C * N * K2 * 4));
     HIP_TRY(ensure(ctx->wgt, slots * C * 4));
-    HIP_TRY(ensure(ctx->tabs, sizeof(Tabs)));
-    HIP_TRY(ensure(ctx->refxy, refxy.size() * 4));
-    HIP_TRY(ensure(ctx->meta, meta.size() * 4));
-    HIP_TRY(ensure(ctx->gpos, gpos.size() * 2));
     HIP_TRY(hipMemcpyAsync(ctx->tabs.p, &tabs, sizeof tabs, hipMemcpyHostToDevice, s));
     HIP_TRY(hipMemcpyAsync(ctx->refxy.p, refxy.data(), refxy.size() * 4, hipMemcpyHostToDevice, s));
     HIP_TRY(hipMemcpyAsync(ctx->meta.p, meta.data(), meta.size() * 4, hipMemcpyHostToDevice, s));
@@ -432,7 +443,7 @@ int run_step(vbm3d_ctx* ctx, const StepIO& io, hipStream_t s) {
     ag.grid_pos = (const uint16_t*)ctx->gpos.p;
     ag.idx = bm.idx_out; ag.nsx = bm.nsx_out;
     ag.est = fa.est; ag.wgt = fa.wgt; ag.tabs = fa.tabs;
-    ag.R = (int)((p.Ns - 1) / 2 + p.Nf * ((p.Npr - 1) / 2 + io.maxflow));
+    ag.R = member_reach(p, io.maxflow, io.W, io.H);
     ag.divW = make_div32(io.W);
     ag.fallback = io.fallback;
     ag.out = io.out;
@@ -891,7 +902,7 @@ int plan_batch(vbm3d_ctx* ctx, const vbm3d_clip* clips, int n, int nb, const vbm
                         ag.est = d_est + m.est_base;
                         ag.wgt = d_wgt + m.wgt_base;
                         ag.tabs = d_tabs + m.tab;
-                        ag.R = (int)((p.Ns - 1) / 2 + p.Nf * ((p.Npr - 1) / 2 + m.maxflow));
+                        ag.R = member_reach(p, m.maxflow, c.W, c.H);
                         ag.divW = make_div32(c.W);
                         ag.t_begin = m.t_done;
                         ag.fallback = st == 1 ? c.noisy : (nb == 1 ? c.basic : c.noisy);
@@ -1158,7 +1169,9 @@ int vbm3d_run(vbm3d_ctx* ctx, float sigma, float* noisy, const float* fflow, con
     if (!ctx || !noisy || !basic || !denoised || !p1 || !p2) return fail(VBM3D_ERR_ARG, "null argument");
     HIP_TRY(hipSetDevice(ctx->device));
     hipStream_t s = ctx->stream;
-    const size_t n = (size_t)W * H * F * C, nf = (size_t)(F > 1 ? F - 1 : 1) * W * H * 2;
+    // a one-frame clip has no predictive search: its flows are never read
+    if (F <= 1) fflow = bflow = nullptr;
+    const size_t n = (size_t)W * H * F * C, nf = (size_t)(F > 1 ? F - 1 : 0) * W * H * 2;
     // With step 1 on, the noisy video is uploaded in chunks while step 1
     // runs on the frames already there (run_device's HostUpload).
     const bool chunked = p1->k > 0;
@@ -1216,7 +1229,9 @@ int vbm3d_step(vbm3d_ctx* ctx, int step, float sigma, const float* noisy, const 
         return fail(VBM3D_ERR_ARG, "invalid argument");
     HIP_TRY(hipSetDevice(ctx->device));
     hipStream_t s = ctx->stream;
-    const size_t n = (size_t)W * H * F * C, nf = (size_t)(F > 1 ? F - 1 : 1) * W * H * 2;
+    // a one-frame clip has no predictive search: its flows are never read
+    if (F <= 1) fflow = bflow = nullptr;
+    const size_t n = (size_t)W * H * F * C, nf = (size_t)(F > 1 ? F - 1 : 0) * W * H * 2;
     int rc = check(*prm, W, H, F, C, step);
     if (rc) return rc;
     HIP_TRY(upload(ctx->noisy, noisy, n, s));
@@ -1310,7 +1325,9 @@ int vbm3d_block_match(vbm3d_ctx* ctx, const float* vid, unsigned W, unsigned H, 
     if (nrefs == 0) return VBM3D_OK;
     HIP_TRY(hipSetDevice(ctx->device));
     hipStream_t s = ctx->stream;
-    const size_t n = (size_t)W * H * F * C, nf = (size_t)(F > 1 ? F - 1 : 1) * W * H * 2;
+    // a one-frame clip has no predictive search: its flows are never read
+    if (F <= 1) fflow = bflow = nullptr;
+    const size_t n = (size_t)W * H * F * C, nf = (size_t)(F > 1 ? F - 1 : 0) * W * H * 2;
     HIP_TRY(upload(ctx->noisy, vid, n, s));
     if (fflow) HIP_TRY(upload(ctx->ff, fflow, nf, s));
     if (bflow) HIP_TRY(upload(ctx->bf, bflow, nf, s));

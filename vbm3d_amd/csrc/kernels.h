@@ -1,5 +1,6 @@
 // kernels.h -- device-side argument blocks and launch helpers shared by
-// the HIP kernels (kernels.hip) and the host driver (capi.hip).
+// the HIP kernels (kernels_bm.hip, kernels_filter.hip, kernels_agg.hip)
+// and the host driver (capi.hip).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -171,7 +172,7 @@ struct WorkQueue {
     int grab;                     // items taken per atomic
 };
 
-// Launchers (kernels.hip).  Return hipSuccess or the launch error.
+// Launchers (kernels_*.hip).  Return hipSuccess or the launch error.
 // 1: the search shape the kernels also hold as compile-time constants
 // (Ns 7, Npr 5, Nb 2: the default parameters of both steps); 0 otherwise.
 inline int bm_shape(const BMArgs& a) { return a.Ns == 7 && a.Npr == 5 && a.Nb == 2 ? 1 : 0; }
