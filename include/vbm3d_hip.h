@@ -103,6 +103,28 @@ int vbm3d_run_device(vbm3d_ctx* ctx, float sigma, float* d_noisy,
                      int color_space, int ref_grid_nb, float* d_basic,
                      float* d_denoised, void* stream);
 
+/* vbm3d_run_device for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  No counterpart in the reference.  Same arguments and
+ * results as vbm3d_run_device, without a context: runs on the current
+ * device, enqueued on `stream` (a hipStream_t; NULL = the null stream) and
+ * synchronised before return.  It calls no hipMalloc, hipFree or other
+ * allocating routine: the per-patch rings, transform tables and the flow
+ * maximum live in a 256 MiB device array of the library, which each GPU
+ * the library runs on holds from the first call on it until the process
+ * ends.  Clips whose rings at the smallest chunking (2 Nf + 1 frames of
+ * reference patches, N k^2 C + N + C + 1 words each) do not fit are
+ * refused with VBM3D_ERR_UNSUPPORTED before anything is enqueued; at the
+ * default parameters that admits e.g. 320x240 grey or 176x144 RGB clips
+ * of any length.  Calls that overlap on one GPU share that GPU's array and
+ * race: serialise them. */
+int vbm3d_run_device_noalloc(float sigma, float* d_noisy, const float* d_fflow,
+                             const float* d_bflow, unsigned W, unsigned H,
+                             unsigned F, unsigned C, const vbm3d_params* p1,
+                             const vbm3d_params* p2, int color_space,
+                             int ref_grid_nb, float* d_basic,
+                             float* d_denoised, void* stream);
+
 /* Predictive block matching only (computeSimilarPatches,
  * src/vbm3d.cpp:1082-1197) for nrefs reference patches given by index
  * t*W*H*C + y*W + x, on host buffers.  Per ref: N sorted indices and

@@ -17,6 +17,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <functional>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -47,8 +48,28 @@ struct DevBuf {
     size_t cap = 0;
 };
 
+// Fixed device memory that ensure() carves buffers from instead of
+// allocating, while vbm3d_run_device_noalloc runs on this thread.
+struct Arena {
+    char* base = nullptr;  // device address of the scratch array
+    size_t size = 0, used = 0, fixed = 0;  // fixed: carved once per call (kept across steps)
+    bool overflow = false;
+};
+thread_local Arena* g_arena = nullptr;
+
 hipError_t ensure(DevBuf& b, size_t bytes) {
     if (bytes <= b.cap) return hipSuccess;
+    if (g_arena) {
+        const size_t at = (g_arena->used + 255) & ~(size_t)255;
+        if (at + bytes > g_arena->size) {
+            g_arena->overflow = true;
+            return hipErrorOutOfMemory;
+        }
+        b.p = g_arena->base + at;
+        b.cap = bytes;
+        g_arena->used = at + bytes;
+        return hipSuccess;
+    }
     if (b.p) {
         hipError_t e = hipFree(b.p);
         if (e != hipSuccess) return e;
@@ -332,6 +353,7 @@ hipError_t timed(vbm3d_ctx* ctx, int kernel, int step, hipStream_t s, F&& launch
 // Bytes the per-patch rings may take: VBM3D_EST_BUDGET_MB when set, else
 // 60 % of the device memory that is free or already held by the rings.
 size_t ring_budget(const vbm3d_ctx* ctx, size_t held) {
+    if (g_arena) return g_arena->size - std::min(g_arena->size, g_arena->used + 4096);
     if (ctx->est_budget) return ctx->est_budget;
     size_t fr = 0, tot = 0;
     if (hipMemGetInfo(&fr, &tot) != hipSuccess) return (size_t)32 << 30;
@@ -365,12 +387,20 @@ int run_step(vbm3d_ctx* ctx, const StepIO& io, hipStream_t s) {
     const unsigned Fr = io.F - p.kt + 1;
     Tabs tabs;
     fill_tabs(tabs, K);
+    if (g_arena) {
+        // the scratch array: this step's buffers start where the call's
+        // fixed ones end (the previous step's are no longer read)
+        g_arena->used = g_arena->fixed;
+        for (DevBuf* b : {&ctx->idx, &ctx->nsx, &ctx->est, &ctx->wgt, &ctx->tabs, &ctx->refxy, &ctx->meta, &ctx->gpos})
+            *b = DevBuf{};
+    }
     HIP_TRY(ensure(ctx->tabs, sizeof(Tabs)));
     HIP_TRY(ensure(ctx->refxy, refxy.size() * 4));
     HIP_TRY(ensure(ctx->meta, meta.size() * 4));
     HIP_TRY(ensure(ctx->gpos, gpos.size() * 2));
     // chunk size from the estimate budget; ring holds S + 2 Nf frames
-    const size_t per_frame = (size_t)nrf * C * N * K2 * sizeof(float);
+    const size_t per_frame = g_arena ? (size_t)nrf * (C * N * K2 + N + 1 + C) * sizeof(float)
+                                     : (size_t)nrf * C * N * K2 * sizeof(float);
     const size_t budget = ring_budget(ctx, ctx->est.cap + ctx->idx.cap + ctx->nsx.cap + ctx->wgt.cap);
     size_t cap_frames = std::max<size_t>(1, budget / std::max<size_t>(per_frame, 1));
     unsigned S = (unsigned)std::max<long>(1, (long)cap_frames - 2 * (long)p.Nf);
@@ -1053,6 +1083,59 @@ std::string batch_key(const vbm3d_clip* clips, int n, int nb, const vbm3d_batch_
 
 }  // namespace
 
+// ---------------------------------------------------------------------
+// vbm3d_run_device_noalloc: the device-buffer entry without allocation.
+// Its per-patch rings, tables and flow maximum live in `scratch`, a
+// file-scope device array that every GPU holds from the library's first
+// use on it until the process ends.  Host code never passes `scratch`
+// itself (its name is a host-side stand-in, not a device address): it
+// takes the array's address per device with hipGetSymbolAddress once and
+// uses that.
+// ---------------------------------------------------------------------
+constexpr size_t kScratchBytes = (size_t)256 << 20;
+__device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+
+std::mutex g_scratch_mu;
+std::vector<char*> g_scratch_ptr;  // device address of `scratch`, per device
+
+// Address of this device's `scratch` (dev must be current), looked up on
+// the first call on the device.
+int scratch_address(int dev, char** out) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (dev < 0) return fail(VBM3D_ERR_ARG, "no current device");
+    if ((int)g_scratch_ptr.size() <= dev) g_scratch_ptr.resize((size_t)dev + 1, nullptr);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(VBM3D_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_ptr[dev] = (char*)p;
+    }
+    *out = g_scratch_ptr[dev];
+    return VBM3D_OK;
+}
+
+// Scratch one step needs at its smallest chunking (rings of min(Fr,
+// 2 Nf + 1) frames), as run_step carves it; 0 when the grid is invalid
+// (run_step then reports the error).
+size_t step_scratch_bytes(const vbm3d_params& p, unsigned W, unsigned H, unsigned F, unsigned C, int nb) {
+    GridDef g;
+    if (p.k == 0 || F < p.kt || !build_grid(g, W, H, p.k, p.p, p.n, nb)) return 0;
+    size_t nrf = 0, npos = 0;
+    for (size_t c = 0; c < g.rows.size(); c++) {
+        nrf += g.rows[c].size() * g.cols[c].size();
+        npos += g.rows[c].size() + g.cols[c].size();
+    }
+    auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    const size_t Fr = F - p.kt + 1, ring = std::min<size_t>(Fr, 1 + 2 * (size_t)p.Nf), slots = ring * nrf;
+    return al(sizeof(Tabs)) + al(nrf * 4) + al(g.rows.size() * 5 * 4) + al(npos * 2) +
+           slots * ((size_t)C * p.N * p.k * p.k + p.N + 1 + C) * 4 + 4096;
+}
+
+}  // namespace
+
 extern "C" {
 
 const char* vbm3d_last_error(void) { return g_err.c_str(); }
@@ -1149,6 +1232,43 @@ long vbm3d_grid(unsigned W, unsigned H, unsigned k, unsigned p, int n, int nb, u
                 cnt++;
             }
     return cnt;
+}
+
+int vbm3d_run_device_noalloc(float sigma, float* d_noisy, const float* d_fflow, const float* d_bflow,
+                             unsigned W, unsigned H, unsigned F, unsigned C, const vbm3d_params* p1,
+                             const vbm3d_params* p2, int color_space, int ref_grid_nb, float* d_basic,
+                             float* d_denoised, void* stream) {
+    if (!d_noisy || !d_basic || !d_denoised || !p1 || !p2) return fail(VBM3D_ERR_ARG, "null argument");
+    int dev = -1;
+    HIP_TRY(hipGetDevice(&dev));
+    char* base = nullptr;
+    int rc = scratch_address(dev, &base);
+    if (rc) return rc;
+    size_t need = 256;  // the flow maximum
+    for (int st = 1; st <= 2; st++) {
+        const vbm3d_params& p = st == 1 ? *p1 : *p2;
+        if (p.k > 0) need = std::max(need, 256 + step_scratch_bytes(p, W, H, F, C, ref_grid_nb));
+    }
+    if (need > kScratchBytes) {
+        char b[200];
+        std::snprintf(b, sizeof b, "this clip needs %.1f MiB of scratch; vbm3d_run_device_noalloc has %zu MiB",
+                      (double)need / (1 << 20), kScratchBytes >> 20);
+        return fail(VBM3D_ERR_UNSUPPORTED, b);
+    }
+    vbm3d_ctx ctx;  // no stream and no buffers of its own
+    ctx.device = dev;
+    Arena arena;
+    arena.base = base;
+    arena.size = kScratchBytes;
+    g_arena = &arena;
+    hipError_t e = ensure(ctx.scal, 4);  // kept across both steps
+    arena.fixed = arena.used;
+    rc = e == hipSuccess ? run_device(&ctx, sigma, d_noisy, d_fflow, d_bflow, W, H, F, C, p1, p2, color_space,
+                                      ref_grid_nb, d_basic, d_denoised, (hipStream_t)stream)
+                         : fail(VBM3D_ERR_HIP, "scratch");
+    g_arena = nullptr;
+    if (rc && arena.overflow) return fail(VBM3D_ERR_UNSUPPORTED, "the clip does not fit the no-allocation scratch");
+    return rc;
 }
 
 int vbm3d_run_device(vbm3d_ctx* ctx, float sigma, float* d_noisy, const float* d_fflow,
