@@ -20,7 +20,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libvbm3d_hip.so")
 # one translation unit per kernel family, compiled in parallel
 SOURCES = [os.path.join(CSRC, f) for f in ("kernels_bm.hip", "kernels_filter.hip", "kernels_agg.hip",
-                                           "capi.hip")]
+                                           "kernels_gen.hip", "capi.hip")]
 OBJDIR = os.path.join(HERE, "_build")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("VBM3D_ARCH", "gfx950")

@@ -184,10 +184,17 @@ float kaiser_at(unsigned k, unsigned a, unsigned b) {
                                    {0.4055f, 0.6640f, 0.8544f, 0.9243f}, {0.4387f, 0.7184f, 0.9243f, 1.0000f}};
     static const float q8[4][4] = {{0.1924f, 0.2989f, 0.3846f, 0.4325f}, {0.2989f, 0.4642f, 0.5974f, 0.6717f},
                                    {0.3846f, 0.5974f, 0.7688f, 0.8644f}, {0.4325f, 0.6717f, 0.8644f, 0.9718f}};
+    static const float q12[6][6] = {{0.1924f, 0.2615f, 0.3251f, 0.3782f, 0.4163f, 0.4362f},
+                                    {0.2615f, 0.3554f, 0.4419f, 0.5139f, 0.5657f, 0.5927f},
+                                    {0.3251f, 0.4419f, 0.5494f, 0.6390f, 0.7033f, 0.7369f},
+                                    {0.3782f, 0.5139f, 0.6390f, 0.7433f, 0.8181f, 0.8572f},
+                                    {0.4163f, 0.5657f, 0.7033f, 0.8181f, 0.9005f, 0.9435f},
+                                    {0.4362f, 0.5927f, 0.7369f, 0.8572f, 0.9435f, 0.9885f}};
     if (k == 4) return q4[a][b];
     if (k == 6) return q6[a][b];
     if (k == 7) return q7[a][b];
     if (k == 8) return q8[a][b];
+    if (k == 12) return q12[a][b];
     return 1.0f;
 }
 
@@ -225,7 +232,63 @@ void fill_tabs(Tabs& t, unsigned k) {
     t.dct_out = 1.0f / (float)(k * 2);
 }
 
-// What the kernels implement; everything else is rejected with a message.
+// Tables of the general path: k up to 16, and the temporal DCT of length
+// kt (src/vbm3d.cpp:1866-1956: REDFT10 / REDFT01 in closed form, double
+// then one rounding to float; 1/sqrt(2 kt) rounded to float).
+void fill_tabs_gen(TabsG& t, unsigned k, unsigned kt) {
+    std::memset(&t, 0, sizeof t);
+    Tabs b;
+    fill_tabs(b, std::min(k, 8u));
+    std::memcpy(t.biorF, b.biorF, sizeof t.biorF);
+    std::memcpy(t.biorI, b.biorI, sizeof t.biorI);
+    for (unsigned v = 0; v < k; v++)
+        for (unsigned q = 0; q < k; q++) {
+            t.dctF[v * k + q] = (float)(2.0 * std::cos(M_PI * (2.0 * q + 1.0) * v / (2.0 * k)));
+            t.dctI[q * k + v] = v == 0 ? 1.0f : (float)(2.0 * std::cos(M_PI * v * (2.0 * q + 1.0) / (2.0 * k)));
+        }
+    const float coef = 0.5f / ((float)k);
+    for (unsigned i = 0; i < k; i++)
+        for (unsigned j = 0; j < k; j++) {
+            if (i == 0 && j == 0) {
+                t.nrm[i * k + j] = 0.5f * coef;
+                t.inv[i * k + j] = 2.0f;
+            } else if (i * j == 0) {
+                t.nrm[i * k + j] = (float)(0.7071067811865475 * coef);
+                t.inv[i * k + j] = (float)1.414213562373095;
+            } else {
+                t.nrm[i * k + j] = 1.0f * coef;
+                t.inv[i * k + j] = 1.0f;
+            }
+        }
+    for (unsigned p = 0; p < k; p++)
+        for (unsigned q = 0; q < k; q++)
+            t.kaiser[p * k + q] = kaiser_at(k, std::min(p, k - 1 - p), std::min(q, k - 1 - q));
+    t.dct_out = 1.0f / (float)(k * 2);
+    for (unsigned u = 0; u < kt; u++)
+        for (unsigned tt = 0; tt < kt; tt++) {
+            t.tF[u * kt + tt] = (float)(2.0 * std::cos(M_PI * (2.0 * tt + 1.0) * u / (2.0 * kt)));
+            t.tI[tt * kt + u] = u == 0 ? 1.0f : (float)(2.0 * std::cos(M_PI * u * (2.0 * tt + 1.0) / (2.0 * kt)));
+        }
+    t.tnorm = (float)(1. / std::sqrt(2.0 * kt));
+}
+
+// Whether the fast kernels (kernels_bm/filter/agg.hip) implement these
+// parameters; the general kernels (kernels_gen.hip) take the rest.
+bool fast_ok(const vbm3d_params& p, unsigned W, unsigned H) {
+    if (p.kt != 1 || p.k < 4 || p.k > 8) return false;
+    if (p.N == 0 || p.N > (unsigned)kMaxN || (p.N & (p.N - 1))) return false;
+    if (p.Nb == 0 || p.Nb > 8 || p.Nb * p.Nb > 64) return false;
+    if (p.Ns == 0 || p.Ns > (unsigned)kMaxWin || p.Npr == 0 || p.Npr > (unsigned)kMaxWin) return false;
+    if (2 * p.Nb * (unsigned)pow2ceil((int)(p.Npr * p.Npr)) > (unsigned)kMaxSlots) return false;
+    if (p.Nb * (2 * p.Nf + 1) > (unsigned)kMaxBest) return false;
+    // every predictive window must yield Nb candidates: the fast matching
+    // keeps exactly Nb picks per frame
+    const unsigned h = (p.Npr - 1) / 2 + 1;
+    return std::min(W - p.k + 1, h) * std::min(H - p.k + 1, h) >= p.Nb;
+}
+
+// What the kernels implement (fast or general); everything else is
+// rejected with a message.
 int check(const vbm3d_params& p, unsigned W, unsigned H, unsigned F, unsigned C, int step) {
     char b[160];
     auto no = [&](int code, const char* m) {
@@ -237,27 +300,27 @@ int check(const vbm3d_params& p, unsigned W, unsigned H, unsigned F, unsigned C,
     if (W > 65535 || H > 65535) return no(VBM3D_ERR_ARG, "frame dimensions above 65535");
     if ((unsigned long long)W * H * C * F >= (1ull << 32))
         return no(VBM3D_ERR_ARG, "video too large for 32-bit patch indices");
-    if (p.k < 4 || p.k > 8) return no(VBM3D_ERR_UNSUPPORTED, "patch size k must be in [4, 8]");
+    if (p.k < 1 || p.k > (unsigned)kGenMaxK) return no(VBM3D_ERR_UNSUPPORTED, "patch size k must be in [1, 16]");
     if (W < p.k || H < p.k) return no(VBM3D_ERR_ARG, "frame smaller than a patch");
-    if (p.kt != 1) return no(VBM3D_ERR_UNSUPPORTED, "only kt == 1 is supported");
-    if (p.N == 0 || p.N > (unsigned)kMaxN || (p.N & (p.N - 1)))
-        return no(VBM3D_ERR_UNSUPPORTED, "N must be a power of two <= 16");
-    if (p.Nb == 0 || p.Nb > 8) return no(VBM3D_ERR_UNSUPPORTED, "Nb must be in [1, 8]");
-    if (p.Ns == 0 || p.Ns > (unsigned)kMaxWin || p.Npr == 0 || p.Npr > (unsigned)kMaxWin)
-        return no(VBM3D_ERR_UNSUPPORTED, "Ns and Npr must be in [1, 8]");
-    if (2 * p.Nb * (unsigned)pow2ceil((int)(p.Npr * p.Npr)) > (unsigned)kMaxSlots)
-        return no(VBM3D_ERR_UNSUPPORTED, "2 Nb Npr^2 (Npr^2 rounded up to a power of two) > 512");
-    if (p.Nb * p.Nb > 64) return no(VBM3D_ERR_UNSUPPORTED, "Nb^2 > 64");
-    if (p.Nb * (2 * p.Nf + 1) > (unsigned)kMaxBest) return no(VBM3D_ERR_UNSUPPORTED, "Nb * (2 Nf + 1) > 64");
+    if (p.kt < 1 || p.kt > (unsigned)kGenMaxKt) return no(VBM3D_ERR_UNSUPPORTED, "kt must be in [1, 8]");
+    if (p.kt > F) return no(VBM3D_ERR_ARG, "fewer frames than kt");
+    if (p.k * p.k * p.kt > (unsigned)kGenMaxK2t) return no(VBM3D_ERR_UNSUPPORTED, "k^2 kt > 256");
+    if (p.N == 0 || p.N > (unsigned)kGenMaxN) return no(VBM3D_ERR_UNSUPPORTED, "N must be in [1, 32]");
+    if (p.Nb == 0 || p.Nb > (unsigned)kGenMaxNb) return no(VBM3D_ERR_UNSUPPORTED, "Nb must be in [1, 16]");
+    if (p.Ns == 0 || p.Ns > (unsigned)kGenMaxWin || p.Npr == 0 || p.Npr > (unsigned)kGenMaxWin)
+        return no(VBM3D_ERR_UNSUPPORTED, "Ns and Npr must be in [1, 15]");
+    if (p.Nf > (unsigned)kGenMaxNf) return no(VBM3D_ERR_UNSUPPORTED, "Nf must be <= 8");
     if (p.p == 0) return no(VBM3D_ERR_ARG, "step p must be > 0");
     if (p.T_2D != VBM3D_DCT && p.T_2D != VBM3D_BIOR) return no(VBM3D_ERR_ARG, "T_2D must be DCT (0) or BIOR (1)");
     if (p.T_2D == VBM3D_BIOR && p.k != 8) return no(VBM3D_ERR_ARG, "BIOR needs k == 8");
-    // each search window must yield Nb candidates (src/vbm3d.cpp:1115-1119)
-    for (unsigned s : {p.Ns, p.Npr}) {
-        const unsigned h = (s - 1) / 2 + 1;
-        if (std::min(W - p.k + 1, h) * std::min(H - p.k + 1, h) < p.Nb)
-            return no(VBM3D_ERR_UNSUPPORTED, "search window smaller than Nb");
-    }
+    // the reference-frame window must yield Nb candidates (src/vbm3d.cpp:1115-1119)
+    const unsigned h = (p.Ns - 1) / 2 + 1;
+    if (std::min(W - p.k + 1, h) * std::min(H - p.k + 1, h) < p.Nb)
+        return no(VBM3D_ERR_UNSUPPORTED, "search window smaller than Nb");
+    if (!fast_ok(p, W, H) && (size_t)gen_bm_layout(p.Nb, p.Nf, p.Ns, p.Npr).total * 4 > 64 * 1024)
+        return no(VBM3D_ERR_UNSUPPORTED, "search shape too large for the general matching kernel");
+    if (!fast_ok(p, W, H) && (size_t)gen_filter_words((int)p.k, (int)p.kt, (int)p.N, 2) * 4 > 64 * 1024)
+        return no(VBM3D_ERR_UNSUPPORTED, "k^2 kt N too large for the general filtering kernel (2 k^2 kt N + 3 k^2 + N kt words <= 64 KiB)");
     return VBM3D_OK;
 }
 
@@ -268,7 +331,10 @@ int check(const vbm3d_params& p, unsigned W, unsigned H, unsigned F, unsigned C,
 // no member lies farther), so that a huge or non-finite flow value cannot
 // wrap it.
 int member_reach(const vbm3d_params& p, unsigned maxflow, unsigned W, unsigned H) {
-    const unsigned long long r = (p.Ns - 1) / 2 + (unsigned long long)p.Nf * ((p.Npr - 1) / 2 + (unsigned long long)maxflow);
+    unsigned long long r = (p.Ns - 1) / 2 + (unsigned long long)p.Nf * ((p.Npr - 1) / 2 + (unsigned long long)maxflow);
+    // a motion-compensated trajectory drifts by at most the rounded sum of
+    // kt - 1 flow steps
+    if (p.mc && p.kt > 1) r += (unsigned long long)(p.kt - 1) * ((unsigned long long)maxflow + 1);
     return (int)std::min<unsigned long long>(r, std::max(W, H));
 }
 
@@ -290,6 +356,7 @@ struct vbm3d_ctx {
     hipStream_t side = nullptr;
     hipEvent_t ready[3] = {nullptr, nullptr, nullptr};  // noisy final, basic final, upload chunk
     DevBuf idx, nsx, nsim, dist, est, wgt, tabs, refxy, meta, gpos, rlist, scal, stats, counts;
+    DevBuf tabsg, mpos;  // general path: tables, members' slice corners
     size_t est_budget = 0;  // bytes for the per-patch rings; 0 = from free device memory
     unsigned chunk_frames = 0;  // per-clip path: frames per chunk; 0 = from the budget
     // kernel profiling (vbm3d_profile / vbm3d_profile_read)
@@ -385,37 +452,50 @@ int run_step(vbm3d_ctx* ctx, const StepIO& io, hipStream_t s) {
     const unsigned nrf = (unsigned)refxy.size();
     const unsigned K = p.k, K2 = K * K, C = io.C, N = p.N;
     const unsigned Fr = io.F - p.kt + 1;
+    const bool gen = !fast_ok(p, io.W, io.H);  // the general kernels
+    if (gen && io.adjoint)
+        return fail(VBM3D_ERR_UNSUPPORTED, "the step-2 adjoint runs the fast kernels' parameter sets only");
+    const unsigned kt = p.kt, K2t = K2 * kt;
     Tabs tabs;
-    fill_tabs(tabs, K);
+    fill_tabs(tabs, std::min(K, 8u));
+    TabsG tabsg;
+    if (gen) fill_tabs_gen(tabsg, K, kt);
     if (g_arena) {
         // the scratch array: this step's buffers start where the call's
         // fixed ones end (the previous step's are no longer read)
         g_arena->used = g_arena->fixed;
-        for (DevBuf* b : {&ctx->idx, &ctx->nsx, &ctx->est, &ctx->wgt, &ctx->tabs, &ctx->refxy, &ctx->meta, &ctx->gpos})
+        for (DevBuf* b : {&ctx->idx, &ctx->nsx, &ctx->est, &ctx->wgt, &ctx->tabs, &ctx->refxy, &ctx->meta, &ctx->gpos,
+                          &ctx->tabsg, &ctx->mpos})
             *b = DevBuf{};
     }
     HIP_TRY(ensure(ctx->tabs, sizeof(Tabs)));
+    if (gen) HIP_TRY(ensure(ctx->tabsg, sizeof(TabsG)));
     HIP_TRY(ensure(ctx->refxy, refxy.size() * 4));
     HIP_TRY(ensure(ctx->meta, meta.size() * 4));
     HIP_TRY(ensure(ctx->gpos, gpos.size() * 2));
     // chunk size from the estimate budget; ring holds S + 2 Nf frames
-    const size_t per_frame = g_arena ? (size_t)nrf * (C * N * K2 + N + 1 + C) * sizeof(float)
-                                     : (size_t)nrf * C * N * K2 * sizeof(float);
+    // the general path's members span kt frames, so the ring keeps kt - 1
+    // more frames of reference patches for the aggregation
+    const size_t mpos_words = gen && kt > 1 ? (size_t)N * kt : 0;
+    const size_t per_frame = g_arena ? (size_t)nrf * (C * N * K2t + N + 1 + C + mpos_words) * sizeof(float)
+                                     : (size_t)nrf * C * N * K2t * sizeof(float);
     const size_t budget = ring_budget(ctx, ctx->est.cap + ctx->idx.cap + ctx->nsx.cap + ctx->wgt.cap);
     size_t cap_frames = std::max<size_t>(1, budget / std::max<size_t>(per_frame, 1));
-    unsigned S = (unsigned)std::max<long>(1, (long)cap_frames - 2 * (long)p.Nf);
+    unsigned S = (unsigned)std::max<long>(1, (long)cap_frames - 2 * (long)p.Nf - (long)(kt - 1));
     if (ctx->chunk_frames) S = std::min(S, ctx->chunk_frames);
     if (io.max_chunk) S = std::min(S, io.max_chunk);
-    unsigned ring = std::min<unsigned>(Fr, S + 2 * p.Nf);
+    unsigned ring = std::min<unsigned>(Fr, S + 2 * p.Nf + (kt - 1));
     if (S >= Fr) { S = Fr; ring = Fr; }
     const size_t slots = (size_t)ring * nrf;
     if ((slots * kMaxN) >> 32) return fail(VBM3D_ERR_UNSUPPORTED, "too many reference patches in flight");
 
     HIP_TRY(ensure(ctx->idx, slots * N * 4));
     HIP_TRY(ensure(ctx->nsx, slots * 4));
-    HIP_TRY(ensure(ctx->est, slots * C * N * K2 * 4));
+    HIP_TRY(ensure(ctx->est, slots * C * N * K2t * 4));
     HIP_TRY(ensure(ctx->wgt, slots * C * 4));
+    if (mpos_words) HIP_TRY(ensure(ctx->mpos, slots * mpos_words * 4));
     HIP_TRY(hipMemcpyAsync(ctx->tabs.p, &tabs, sizeof tabs, hipMemcpyHostToDevice, s));
+    if (gen) HIP_TRY(hipMemcpyAsync(ctx->tabsg.p, &tabsg, sizeof tabsg, hipMemcpyHostToDevice, s));
     HIP_TRY(hipMemcpyAsync(ctx->refxy.p, refxy.data(), refxy.size() * 4, hipMemcpyHostToDevice, s));
     HIP_TRY(hipMemcpyAsync(ctx->meta.p, meta.data(), meta.size() * 4, hipMemcpyHostToDevice, s));
     HIP_TRY(hipMemcpyAsync(ctx->gpos.p, gpos.data(), gpos.size() * 2, hipMemcpyHostToDevice, s));
@@ -481,6 +561,27 @@ int run_step(vbm3d_ctx* ctx, const StepIO& io, hipStream_t s) {
     ag.den_out = io.den_raw;
     ag.raw = io.adjoint ? 1 : 0;
 
+    GenArgs ga{};
+    if (gen) {
+        ga.vid = bm.vid;
+        ga.noisy = io.noisy;
+        ga.basic = io.basic;
+        ga.ff = io.ff;
+        ga.bf = io.bf;
+        ga.W = io.W; ga.H = io.H; ga.F = io.F; ga.C = C; ga.wh = bm.wh; ga.whc = bm.whc;
+        ga.nrf = nrf; ga.ring = ring;
+        ga.ref_xy = bm.ref_xy;
+        ga.K = (int)K; ga.kt = (int)kt; ga.mc = p.mc ? 1 : 0;
+        ga.Ns = p.Ns; ga.Npr = p.Npr; ga.Nb = p.Nb; ga.Nf = p.Nf; ga.N = p.N;
+        ga.d = p.d; ga.tau = p.tau;
+        ga.T2D = p.T_2D; ga.T3D = p.T_3D; ga.lambda = p.lambda3D;
+        for (int c = 0; c < 3; c++) ga.sig[c] = sig[c];
+        ga.idx_out = bm.idx_out; ga.nsx_out = bm.nsx_out; ga.count_out = bm.count_out;
+        ga.tabs = (const TabsG*)ctx->tabsg.p;
+        ga.est = fa.est; ga.wgt = fa.wgt;
+        ga.mpos = mpos_words ? (uint32_t*)ctx->mpos.p : nullptr;
+    }
+
     unsigned t_done = 0;
     for (unsigned a = 0; a < Fr; a += S) {
         const unsigned e = std::min(Fr, a + S);
@@ -491,18 +592,25 @@ int run_step(vbm3d_ctx* ctx, const StepIO& io, hipStream_t s) {
             rc = io.need_frames(std::min(io.F, e + p.Nf + p.kt - 1));
             if (rc) return rc;
         }
-        HIP_TRY(timed(ctx, 0, io.step, s, [&] { return launch_bm((int)K, bm, s); }));
+        ga.r_begin = bm.r_begin;
+        ga.r_end = bm.r_end;
+        HIP_TRY(timed(ctx, 0, io.step, s, [&] { return gen ? launch_bm_gen(ga, s) : launch_bm((int)K, bm, s); }));
         if (ctx->count)
             HIP_TRY(launch_sum_counts(bm.count_out, bm.r_end - bm.r_begin,
                                       (unsigned long long*)ctx->stats.p + 4 * (io.step - 1), s));
         if (io.adjoint)
             HIP_TRY(launch_filter_adjoint((int)K, fa, s));
         else
-            HIP_TRY(timed(ctx, 1, io.step, s, [&] { return launch_filter((int)K, io.step, fa, s); }));
+            HIP_TRY(timed(ctx, 1, io.step, s, [&] {
+                return gen ? launch_filter_gen(io.step, ga, s) : launch_filter((int)K, io.step, fa, s);
+            }));
         const unsigned ready = e >= Fr ? io.F : (e > p.Nf ? e - p.Nf : 0);
         if (ready > t_done) {
             ag.t_begin = t_done;
-            HIP_TRY(timed(ctx, 2, io.step, s, [&] { return launch_aggregate((int)C, ag, ready - t_done, s); }));
+            HIP_TRY(timed(ctx, 2, io.step, s, [&] {
+                return gen ? launch_aggregate_gen((int)C, ag, ga.mpos, ga.tabs, ready - t_done, s)
+                           : launch_aggregate((int)C, ag, ready - t_done, s);
+            }));
             if (ctx->count) ctx->contributions[io.step] += (double)(ready - t_done) * io.W * io.H * C;
             t_done = ready;
         }
@@ -715,6 +823,11 @@ int plan_batch(vbm3d_ctx* ctx, const vbm3d_clip* clips, int n, int nb, const vbm
             if (p.k == 0) continue;
             int rc = check(p, c.W, c.H, c.F, c.C, st);
             if (rc) return fail(rc, "clip " + std::to_string(i) + ": " + g_err);
+            if (!fast_ok(p, c.W, c.H))
+                return fail(VBM3D_ERR_UNSUPPORTED, "clip " + std::to_string(i) + ", step " + std::to_string(st) +
+                                                       ": the batched form runs the fast kernels' parameter sets "
+                                                       "(kt 1, k 4..8, N a power of two <= 16, Nb, Ns, Npr <= 8); "
+                                                       "run this clip with vbm3d_run_device");
         }
         int rc = flow_absmax(ctx, c.fflow, c.bflow, c.W, c.H, c.F, s, &maxflow[i]);
         if (rc) return rc;
@@ -1129,9 +1242,11 @@ size_t step_scratch_bytes(const vbm3d_params& p, unsigned W, unsigned H, unsigne
         npos += g.rows[c].size() + g.cols[c].size();
     }
     auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
-    const size_t Fr = F - p.kt + 1, ring = std::min<size_t>(Fr, 1 + 2 * (size_t)p.Nf), slots = ring * nrf;
-    return al(sizeof(Tabs)) + al(nrf * 4) + al(g.rows.size() * 5 * 4) + al(npos * 2) +
-           slots * ((size_t)C * p.N * p.k * p.k + p.N + 1 + C) * 4 + 4096;
+    const bool gen = !fast_ok(p, W, H);
+    const size_t Fr = F - p.kt + 1, ring = std::min<size_t>(Fr, 1 + 2 * (size_t)p.Nf + (p.kt - 1)), slots = ring * nrf;
+    const size_t mpos_words = gen && p.kt > 1 ? (size_t)p.N * p.kt : 0;
+    return al(sizeof(Tabs)) + (gen ? al(sizeof(TabsG)) : 0) + al(nrf * 4) + al(g.rows.size() * 5 * 4) + al(npos * 2) +
+           slots * ((size_t)C * p.N * p.k * p.k * p.kt + p.N + 1 + C + mpos_words) * 4 + 4096;
 }
 
 }  // namespace
@@ -1439,7 +1554,7 @@ int vbm3d_block_match(vbm3d_ctx* ctx, const float* vid, unsigned W, unsigned H, 
     const unsigned whc = W * H * C;
     for (long r = 0; r < nrefs; r++) {
         const unsigned t = ref_idx[r] / whc, rem = ref_idx[r] % whc;
-        if (t >= F || rem >= W * H || rem % W > W - prm->k || rem / W > H - prm->k)
+        if (t + prm->kt > F || rem >= W * H || rem % W > W - prm->k || rem / W > H - prm->k)
             return fail(VBM3D_ERR_ARG, "reference index outside the valid patch range");
     }
     if (nrefs == 0) return VBM3D_OK;
@@ -1473,7 +1588,21 @@ int vbm3d_block_match(vbm3d_ctx* ctx, const float* vid, unsigned W, unsigned H, 
     bm.dist_out = (float*)ctx->dist.p;
     bm.nsim_out = (uint32_t*)ctx->nsim.p;
     bm.nsx_out = (uint32_t*)ctx->nsx.p;
-    HIP_TRY(launch_bm((int)prm->k, bm, s));
+    if (fast_ok(*prm, W, H)) {
+        HIP_TRY(launch_bm((int)prm->k, bm, s));
+    } else {
+        GenArgs ga{};
+        ga.vid = bm.vid; ga.ff = bm.ff; ga.bf = bm.bf;
+        ga.W = W; ga.H = H; ga.F = F; ga.C = C; ga.wh = bm.wh; ga.whc = whc;
+        ga.nrf = 1; ga.ring = 1;
+        ga.ref_list = bm.ref_list;
+        ga.r_begin = 0; ga.r_end = nrefs;
+        ga.K = (int)prm->k; ga.kt = (int)prm->kt; ga.mc = prm->mc ? 1 : 0;
+        ga.Ns = prm->Ns; ga.Npr = prm->Npr; ga.Nb = prm->Nb; ga.Nf = prm->Nf; ga.N = prm->N;
+        ga.d = prm->d; ga.tau = prm->tau;
+        ga.idx_out = bm.idx_out; ga.dist_out = bm.dist_out; ga.nsim_out = bm.nsim_out; ga.nsx_out = bm.nsx_out;
+        HIP_TRY(launch_bm_gen(ga, s));
+    }
     HIP_TRY(hipStreamSynchronize(s));
     HIP_TRY(hipMemcpy(idx_out, ctx->idx.p, nrefs * prm->N * 4, hipMemcpyDeviceToHost));
     HIP_TRY(hipMemcpy(dist_out, ctx->dist.p, nrefs * prm->N * 4, hipMemcpyDeviceToHost));

@@ -172,6 +172,96 @@ struct WorkQueue {
     int grab;                     // items taken per atomic
 };
 
+// ---------------------------------------------------------------------
+// General path (kernels_gen.hip): the parameter sets the fast kernels do
+// not implement, within these bounds.
+// ---------------------------------------------------------------------
+constexpr int kGenMaxK = 16;      // patch size
+constexpr int kGenMaxKt = 8;      // temporal patch size
+constexpr int kGenMaxK2t = 256;   // k^2 kt, coefficients per member and channel
+constexpr int kGenMaxN = 32;      // group size
+constexpr int kGenMaxNb = 16;     // matches per frame
+constexpr int kGenMaxWin = 15;    // Ns, Npr
+constexpr int kGenMaxNf = 8;      // frames searched each way
+constexpr int kGenAggScan = 128;  // reference patches per aggregation scan chunk
+
+// Transform tables of one step of the general path (k up to 16, kt up
+// to 8), uploaded once per step.
+struct TabsG {
+    float dctF[256];   // REDFT10 [v][q], row stride k
+    float dctI[256];   // REDFT01 [q][v]
+    float nrm[256];    // coef_norm
+    float inv[256];    // coef_norm_inv
+    float kaiser[256];
+    float biorF[64], biorI[64];
+    float tF[64], tI[64];  // temporal REDFT10 [u][t], REDFT01 [t][u], row stride kt
+    float dct_out, tnorm;  // 1/(2k), 1/sqrt(2 kt)
+};
+
+struct GenArgs {
+    const float* vid;    // matching video (noisy in step 1, basic in step 2)
+    const float* noisy;  // filtering: noisy and basic videos
+    const float* basic;
+    const float* ff;     // forward flow [F-1][2][H][W] or null (also the trajectories' flow)
+    const float* bf;     // backward flow or null
+    unsigned W, H, F, C, wh, whc;
+    unsigned nrf, ring;
+    const uint32_t* ref_xy;    // x | y << 16 per in-frame reference patch
+    const uint32_t* ref_list;  // list mode (matching only): global patch indices, slot = r - r_begin
+    long r_begin, r_end;       // global ref ids t*nrf + i (or list positions)
+    int K, kt, mc, Ns, Npr, Nb, Nf, N;
+    float d, tau;
+    int T2D, T3D;
+    float lambda;
+    float sig[3];
+    uint32_t* idx_out;   // [slot][N]
+    float* dist_out;     // [slot][N] or null
+    uint32_t* nsim_out;  // [slot] or null
+    uint32_t* nsx_out;   // [slot]
+    uint32_t* count_out; // null, or per patch: candidates evaluated, group size
+    const TabsG* tabs;
+    float* est;          // [slot][N][C][kt][k^2]
+    float* wgt;          // [slot][C]
+    uint32_t* mpos;      // [slot][N][kt] slice corners x | y << 16, or null (kt == 1)
+};
+
+// Per-wavefront LDS layout of k_bm_gen, in 4-byte words.
+struct GenBMLay {
+    int win, wd, wi, fd, fi, bd, bi, od, oi, chain, rtr, scr, total;
+};
+
+__host__ __device__ inline GenBMLay gen_bm_layout(int Nb, int Nf, int Ns, int Npr) {
+    GenBMLay L;
+    const int ws = (Ns > Npr ? Ns : Npr), wmax = ws * ws;     // window list
+    const int nwin = 1 + 2 * Nf * Nb;                          // windows of one reference patch
+    const int flist = Nb * Nb, blist = Nb * (2 * Nf + 1);     // frame and best lists
+    int smax = wmax > flist ? wmax : flist;
+    smax = smax > blist ? smax : blist;
+    smax = smax > 64 ? smax : 64;
+    int o = 0;
+    L.win = o; o += 5 * nwin;
+    L.wd = o; o += wmax;
+    L.wi = o; o += wmax;
+    L.fd = o; o += flist;
+    L.fi = o; o += flist;
+    L.bd = o; o += blist;
+    L.bi = o; o += blist;
+    L.od = o; o += kGenMaxN;
+    L.oi = o; o += kGenMaxN;
+    L.chain = o; o += 32;
+    L.rtr = o; o += 2 * kGenMaxKt;
+    o = (o + 1) & ~1;  // 8-byte alignment of the heap entries
+    L.scr = o; o += 2 * smax;
+    L.total = o;
+    return L;
+}
+
+// LDS words of k_filter_gen (one wavefront): the group(s) [k^2 kt][N],
+// three k x k scratch tiles and the members' slice corners.
+__host__ __device__ inline int gen_filter_words(int K, int kt, int N, int step) {
+    return K * K * kt * N * (step == 2 ? 2 : 1) + 3 * K * K + N * kt;
+}
+
 // Launchers (kernels_*.hip).  Return hipSuccess or the launch error.
 // 1: the search shape the kernels also hold as compile-time constants
 // (Ns 7, Npr 5, Nb 2: the default parameters of both steps); 0 otherwise.
@@ -199,5 +289,9 @@ int filter_words(int K, int C, int N, int step);
 hipError_t launch_adjoint_fallback(float* gz, const float* gy, const float* den, size_t n, hipStream_t s);
 hipError_t launch_sum_counts(const uint32_t* counts, long n, unsigned long long* stats, hipStream_t s);
 hipError_t launch_flow_absmax(const float* fl, size_t n, unsigned* out, hipStream_t s);
+hipError_t launch_bm_gen(const GenArgs& a, hipStream_t s);
+hipError_t launch_filter_gen(int step, const GenArgs& a, hipStream_t s);
+hipError_t launch_aggregate_gen(int C, const AArgs& a, const uint32_t* mpos, const TabsG* tabs, unsigned frames,
+                                hipStream_t s);
 
 }  // namespace vb

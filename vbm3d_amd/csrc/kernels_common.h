@@ -203,5 +203,78 @@ __device__ __forceinline__ void lds_dma(const void* src, unsigned lds) {
     __builtin_amdgcn_global_load_lds(src, (lds_void*)(size_t)lds, 4, 0, 0);
 }
 
+// ---------------------------------------------------------------------
+// Transforms along the group (registers, NS members).
+// ---------------------------------------------------------------------
+__device__ __forceinline__ float haar_scale(float s) {
+    return (float)((double)s * 0.7071067811865475);  // SQRT2_INV, src/vbm3d.cpp:33
+}
+
+// haar_forward (src/lib_transforms.cpp:453-476) on registers.
+template <int NS>
+__device__ __forceinline__ void haar_fwd(float* v) {
+    float tmp[NS > 1 ? NS : 1];
+#pragma unroll
+    for (int k = NS / 2; k >= 1; k /= 2) {
+#pragma unroll
+        for (int i = 0; i < k; i++) {
+            tmp[i] = haar_scale(v[2 * i] + v[2 * i + 1]);
+            tmp[i + k] = haar_scale(v[2 * i] - v[2 * i + 1]);
+        }
+#pragma unroll
+        for (int i = 0; i < 2 * k; i++) v[i] = tmp[i];
+    }
+}
+
+// haar_inverse (src/lib_transforms.cpp:489-512) on registers.
+template <int NS>
+__device__ __forceinline__ void haar_inv(float* v) {
+    float tmp[NS > 1 ? NS : 1];
+#pragma unroll
+    for (int k = 1; k * 2 <= NS; k *= 2) {
+#pragma unroll
+        for (int i = 0; i < k; i++) {
+            tmp[2 * i] = haar_scale(v[i] + v[i + k]);
+            tmp[2 * i + 1] = haar_scale(v[i] - v[i + k]);
+        }
+#pragma unroll
+        for (int i = 0; i < 2 * k; i++) v[i] = tmp[i];
+    }
+}
+
+// hadamard_transform, unnormalised (src/lib_transforms.cpp:410-440).
+template <int NS>
+__device__ __forceinline__ void hadamard(float* v) {
+    if constexpr (NS == 2) {
+        const float a = v[0], b = v[1];
+        v[0] = a + b;
+        v[1] = a - b;
+    } else if constexpr (NS > 2) {
+        constexpr int h = NS / 2;
+        float tmp[h];
+#pragma unroll
+        for (int k = 0; k < h; k++) {
+            const float a = v[2 * k], b = v[2 * k + 1];
+            v[k] = a + b;
+            tmp[k] = a - b;
+        }
+#pragma unroll
+        for (int k = 0; k < h; k++) v[h + k] = tmp[k];
+        hadamard<h>(v);
+        hadamard<h>(v + h);
+    }
+}
+
+template <int NS>
+__device__ __forceinline__ void t3_fwd(float* v, bool had) {
+    if (had) hadamard<NS>(v);
+    else haar_fwd<NS>(v);
+}
+template <int NS>
+__device__ __forceinline__ void t3_inv(float* v, bool had) {
+    if (had) hadamard<NS>(v);
+    else haar_inv<NS>(v);
+}
+
 }  // namespace
 }  // namespace vb
