@@ -169,3 +169,48 @@ def test_cli_resume_from_basic_and_measure(tmp_path, oracle):
     vals = dict(re.findall(r"-(\w+) = ([0-9.]+)", txt))
     assert float(vals["sigma"]) == 20.0
     assert float(vals["PSNR"]) > float(vals["PSNR_basic"]) - 0.5 > 25.0
+
+
+def write_flo(path, uv):
+    """Middlebury .flo from a planar [2][H][W] flow."""
+    _, h, w = uv.shape
+    with open(path, "wb") as f:
+        np.array([202021.25], "<f4").tofile(f)
+        np.array([w, h], "<i4").tofile(f)
+        np.ascontiguousarray(np.transpose(uv, (1, 2, 0)), "<f4").tofile(f)
+
+
+@pytest.mark.gpu
+def test_cli_png_frames_and_flo_flows(tmp_path, oracle):
+    """The reference's documented invocation on PNG frames (README.txt:70-79)
+    with forward and backward flows from .flo files (-fflow/-bflow,
+    src/main.cpp:353-376): float TIFF output equal to the oracle's run on
+    the same 8-bit frames and flows."""
+    from PIL import Image
+    import vbm3d_amd as V
+    from vbm3d_amd import synth
+    _, noisy = synth.noisy_clip(64, 48, 6, 3, 20.0, 22)
+    vid = np.clip(np.round(noisy), 0, 255).astype(np.uint8)
+    F = vid.shape[0]
+    rng = np.random.default_rng(6)
+    ff = rng.uniform(-2.4, 2.4, (F - 1, 2, 48, 64)).astype(np.float32)
+    bf = rng.uniform(-2.4, 2.4, (F - 1, 2, 48, 64)).astype(np.float32)
+    for t in range(F):
+        Image.fromarray(np.transpose(vid[t], (1, 2, 0)), "RGB").save(tmp_path / f"in_{t:03d}.png")
+    for t in range(F - 1):
+        write_flo(tmp_path / f"ff_{t:03d}.flo", ff[t])          # frame t -> t + 1
+        write_flo(tmp_path / f"bf_{t + 1:03d}.flo", bf[t])      # frame t + 1 -> t
+    r = run_cli(["-i", "in_%03d.png", "-f", "0", "-l", str(F - 1), "-sigma", "20", "-add", "false",
+                 "-fflow", "ff_%03d.flo", "-bflow", "bf_%03d.flo", "-bsic", "", "-deno", "d_%03d.tiff"], tmp_path)
+    assert r.returncode == 0, r.stderr
+    final = np.stack([read_tiff_float(str(tmp_path / f"d_{t:03d}.tiff")) for t in range(F)])
+    _, of, _ = oracle.run_vbm3d(20.0, vid.astype(np.float32), V.initialize_parameters_1(20.0),
+                                V.initialize_parameters_2(20.0), True, 1, fflow=ff, bflow=bf)
+    np.testing.assert_array_equal(final.view(np.uint32), of.view(np.uint32))
+    # PNG output of the same run: iio's float-to-uint8 rule
+    r = run_cli(["-i", "in_%03d.png", "-f", "0", "-l", str(F - 1), "-sigma", "20", "-add", "false",
+                 "-fflow", "ff_%03d.flo", "-bflow", "bf_%03d.flo", "-bsic", "", "-deno", "p_%03d.png"], tmp_path)
+    assert r.returncode == 0, r.stderr
+    png = np.stack([np.transpose(np.asarray(Image.open(tmp_path / f"p_{t:03d}.png"), np.float32), (2, 0, 1))
+                    for t in range(F)])
+    np.testing.assert_array_equal(png, np.clip(np.floor(of.astype(np.float64) + 0.5), 0, 255))

@@ -39,6 +39,7 @@ HOST_LIB = os.path.join(HERE, "libvbm3d_host.so")
 CLI = os.path.join(HERE, "bin", "VBM3Ddenoising")
 CXX = os.environ.get("CXX", "g++")
 CXXFLAGS = ["-O2", "-std=c++17", "-Wall", "-Wextra", f"-I{os.path.join(ROOT, 'include')}"]
+IMG_INC = os.environ.get("VBM3D_IMAGE_INCLUDE", "/opt/conda/include")  # png.h, jpeglib.h
 
 
 def _run(cmd, verbose):
@@ -67,8 +68,11 @@ def build(verbose: bool = False) -> str:
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-o", tmp, *objs], verbose)
     os.replace(tmp, LIB)
     host_src = [os.path.join(HOST, "vbm3d_host.cpp"), os.path.join(HOST, "image_io.cpp")]
-    _run([CXX, *CXXFLAGS, "-fPIC", "-shared", "-o", HOST_LIB + ".tmp", *host_src,
-          f"-L{HERE}", "-l:libvbm3d_hip.so", "-Wl,-rpath,$ORIGIN"], verbose)
+    # PNG through the system's libpng16 (headers of the same 1.6 release
+    # from the image's conda tree, searched after the system's); JPEG's
+    # libjpeg 9 is loaded at run time by image_io.cpp
+    _run([CXX, *CXXFLAGS, "-idirafter", IMG_INC, "-fPIC", "-shared", "-o", HOST_LIB + ".tmp", *host_src,
+          f"-L{HERE}", "-l:libvbm3d_hip.so", "-l:libpng16.so.16", "-ldl", "-Wl,-rpath,$ORIGIN"], verbose)
     os.replace(HOST_LIB + ".tmp", HOST_LIB)
     os.makedirs(os.path.dirname(CLI), exist_ok=True)
     _run([CXX, *CXXFLAGS, "-o", CLI + ".tmp", os.path.join(HOST, "main.cpp"),

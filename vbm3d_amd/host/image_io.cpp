@@ -1,12 +1,19 @@
 // image_io.cpp -- see image_io.h.
 #include "image_io.h"
 
+#include <dlfcn.h>
+#include <setjmp.h>
+
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
 #include <fstream>
+#include <memory>
 #include <sstream>
+
+#include <jpeglib.h>
+#include <png.h>
 
 namespace {
 
@@ -263,6 +270,214 @@ bool save_tiff(const std::string& path, const std::vector<float>& il, unsigned w
     return true;
 }
 
+// ---- PNG (libpng): 8- or 16-bit grey, grey + alpha, RGB or RGBA, any
+// interlacing; palettes and low bit depths expand to 8 bits.  Samples load
+// as their integer values (0-255 or 0-65535), as iio does
+// (src/Utilities/iio.c); saving writes 8 bits, each float v stored as
+// v + 0.5 clamped to [0, 255] and truncated (iio's float-to-uint8 rule,
+// src/Utilities/iio.c:667, 736).
+struct PngMem {
+    const std::vector<uint8_t>* b;
+    size_t pos;
+    jmp_buf jb;
+    char msg[200];
+};
+
+void png_fail(png_structp png, png_const_charp m) {
+    PngMem* io = (PngMem*)png_get_error_ptr(png);
+    std::snprintf(io->msg, sizeof io->msg, "%s", m);
+    longjmp(io->jb, 1);
+}
+
+void png_quiet(png_structp, png_const_charp) {}
+
+void png_read_mem(png_structp png, png_bytep out, png_size_t n) {
+    PngMem* io = (PngMem*)png_get_io_ptr(png);
+    if (io->pos + n > io->b->size()) png_error(png, "truncated PNG");
+    std::memcpy(out, io->b->data() + io->pos, n);
+    io->pos += n;
+}
+
+bool load_png(const std::vector<uint8_t>& b, std::vector<float>& il, unsigned& w, unsigned& h,
+              unsigned& c, std::string& err)
+{
+    if (b.size() < 8 || png_sig_cmp(b.data(), 0, 8)) { err = "not a PNG file"; return false; }
+    // everything libpng may leave half-built when it long-jumps out lives
+    // on the heap, reached through pointers that do not change after setjmp
+    std::unique_ptr<PngMem> io(new PngMem{&b, 0, {}, {0}});
+    std::unique_ptr<std::vector<uint8_t>> buf(new std::vector<uint8_t>);
+    std::unique_ptr<std::vector<png_bytep>> rowv(new std::vector<png_bytep>);
+    png_structp png = png_create_read_struct(PNG_LIBPNG_VER_STRING, io.get(), png_fail, png_quiet);
+    png_infop info = png ? png_create_info_struct(png) : nullptr;
+    if (!png || !info) {
+        png_destroy_read_struct(&png, &info, nullptr);
+        err = "libpng initialisation failed";
+        return false;
+    }
+    std::vector<uint8_t>& data = *buf;
+    std::vector<png_bytep>& rows = *rowv;
+    if (setjmp(io->jb)) {
+        png_destroy_read_struct(&png, &info, nullptr);
+        err = std::string("PNG: ") + io->msg;
+        return false;
+    }
+    png_set_read_fn(png, io.get(), png_read_mem);
+    png_read_info(png, info);
+    png_set_palette_to_rgb(png);
+    png_set_expand_gray_1_2_4_to_8(png);
+    png_set_interlace_handling(png);
+    png_read_update_info(png, info);
+    w = png_get_image_width(png, info);
+    h = png_get_image_height(png, info);
+    c = png_get_channels(png, info);
+    const unsigned depth = png_get_bit_depth(png, info);
+    const size_t rb = png_get_rowbytes(png, info);
+    data.resize(rb * h);
+    rows.resize(h);
+    for (unsigned y = 0; y < h; y++) rows[y] = data.data() + rb * y;
+    png_read_image(png, rows.data());
+    png_read_end(png, nullptr);
+    png_destroy_read_struct(&png, &info, nullptr);
+    il.resize((size_t)w * h * c);
+    for (size_t i = 0; i < il.size(); i++)
+        il[i] = depth == 16 ? (float)((data[2 * i] << 8) | data[2 * i + 1]) : (float)data[i];  // PNG is big-endian
+    return true;
+}
+
+void png_write_mem(png_structp png, png_bytep in, png_size_t n) {
+    std::vector<uint8_t>* out = (std::vector<uint8_t>*)png_get_io_ptr(png);
+    out->insert(out->end(), in, in + n);
+}
+
+void png_flush_mem(png_structp) {}
+
+bool save_png(const std::string& path, const std::vector<float>& il, unsigned w, unsigned h, unsigned c,
+              std::string& err)
+{
+    if (c != 1 && c != 3) { err = path + ": PNG output takes 1 or 3 channels"; return false; }
+    std::vector<uint8_t> px(il.size());
+    for (size_t i = 0; i < il.size(); i++) {
+        const double v = 0.5 + (double)il[i];
+        px[i] = (uint8_t)(v > 0 ? (v < 255 ? v : 255) : 0);
+    }
+    std::unique_ptr<PngMem> io(new PngMem{nullptr, 0, {}, {0}});
+    std::unique_ptr<std::vector<uint8_t>> obuf(new std::vector<uint8_t>);
+    std::unique_ptr<std::vector<png_bytep>> rowv(new std::vector<png_bytep>(h));
+    png_structp png = png_create_write_struct(PNG_LIBPNG_VER_STRING, io.get(), png_fail, png_quiet);
+    png_infop info = png ? png_create_info_struct(png) : nullptr;
+    if (!png || !info) {
+        png_destroy_write_struct(&png, &info);
+        err = "libpng initialisation failed";
+        return false;
+    }
+    std::vector<uint8_t>& out = *obuf;
+    std::vector<png_bytep>& rows = *rowv;
+    if (setjmp(io->jb)) {
+        png_destroy_write_struct(&png, &info);
+        err = path + ": PNG: " + io->msg;
+        return false;
+    }
+    png_set_write_fn(png, &out, png_write_mem, png_flush_mem);
+    png_set_IHDR(png, info, w, h, 8, c == 1 ? PNG_COLOR_TYPE_GRAY : PNG_COLOR_TYPE_RGB, PNG_INTERLACE_NONE,
+                 PNG_COMPRESSION_TYPE_DEFAULT, PNG_FILTER_TYPE_DEFAULT);
+    for (unsigned y = 0; y < h; y++) rows[y] = px.data() + (size_t)y * w * c;
+    png_write_info(png, info);
+    png_write_image(png, rows.data());
+    png_write_end(png, nullptr);
+    png_destroy_write_struct(&png, &info);
+    FILE* f = std::fopen(path.c_str(), "wb");
+    if (!f) { err = path + ": cannot write"; return false; }
+    std::fwrite(out.data(), 1, out.size(), f);
+    std::fclose(f);
+    return true;
+}
+
+// ---- JPEG (read only, as the reference's iio build reads it): libjpeg 9,
+// loaded when the first JPEG frame is read, so that nothing else depends
+// on it.  8-bit grey or RGB samples load as their integer values.
+struct JpegApi {
+    struct jpeg_error_mgr* (*std_error)(struct jpeg_error_mgr*);
+    void (*create)(j_decompress_ptr, int, size_t);
+    void (*mem_src)(j_decompress_ptr, const unsigned char*, unsigned long);
+    int (*read_header)(j_decompress_ptr, boolean);
+    boolean (*start)(j_decompress_ptr);
+    JDIMENSION (*read_lines)(j_decompress_ptr, JSAMPARRAY, JDIMENSION);
+    boolean (*finish)(j_decompress_ptr);
+    void (*destroy)(j_decompress_ptr);
+};
+
+const JpegApi* jpeg_api(std::string& err) {
+    static JpegApi api;
+    static int state = 0;  // 0 not tried, 1 loaded, -1 unavailable
+    if (state == 0) {
+        void* h = nullptr;
+        for (const char* name : {"libjpeg.so.9", "/opt/conda/lib/libjpeg.so.9"})
+            if ((h = dlopen(name, RTLD_NOW | RTLD_LOCAL))) break;
+        state = -1;
+        if (h) {
+            api.std_error = (decltype(api.std_error))dlsym(h, "jpeg_std_error");
+            api.create = (decltype(api.create))dlsym(h, "jpeg_CreateDecompress");
+            api.mem_src = (decltype(api.mem_src))dlsym(h, "jpeg_mem_src");
+            api.read_header = (decltype(api.read_header))dlsym(h, "jpeg_read_header");
+            api.start = (decltype(api.start))dlsym(h, "jpeg_start_decompress");
+            api.read_lines = (decltype(api.read_lines))dlsym(h, "jpeg_read_scanlines");
+            api.finish = (decltype(api.finish))dlsym(h, "jpeg_finish_decompress");
+            api.destroy = (decltype(api.destroy))dlsym(h, "jpeg_destroy_decompress");
+            if (api.std_error && api.create && api.mem_src && api.read_header && api.start && api.read_lines &&
+                api.finish && api.destroy)
+                state = 1;
+        }
+    }
+    if (state != 1) {
+        err = "JPEG input needs libjpeg 9 (libjpeg.so.9)";
+        return nullptr;
+    }
+    return &api;
+}
+
+struct JpegErr {
+    struct jpeg_error_mgr mgr;
+    jmp_buf jb;
+};
+
+void jpeg_fail(j_common_ptr cinfo) { longjmp(((JpegErr*)cinfo->err)->jb, 1); }
+
+bool load_jpeg(const std::vector<uint8_t>& b, std::vector<float>& il, unsigned& w, unsigned& h,
+               unsigned& c, std::string& err)
+{
+    const JpegApi* J = jpeg_api(err);
+    if (!J) return false;
+    // state libjpeg may long-jump out of lives on the heap (see load_png)
+    std::unique_ptr<jpeg_decompress_struct> cip(new jpeg_decompress_struct);
+    std::unique_ptr<JpegErr> je(new JpegErr);
+    std::unique_ptr<std::vector<uint8_t>> buf(new std::vector<uint8_t>);
+    jpeg_decompress_struct& ci = *cip;
+    std::vector<uint8_t>& data = *buf;
+    ci.err = J->std_error(&je->mgr);
+    je->mgr.error_exit = jpeg_fail;
+    if (setjmp(je->jb)) {
+        J->destroy(&ci);
+        err = "corrupt JPEG data";
+        return false;
+    }
+    J->create(&ci, JPEG_LIB_VERSION, sizeof ci);
+    J->mem_src(&ci, b.data(), (unsigned long)b.size());
+    J->read_header(&ci, TRUE);
+    J->start(&ci);
+    data.resize((size_t)ci.output_width * ci.output_height * ci.output_components);
+    while (ci.output_scanline < ci.output_height) {
+        JSAMPROW row = data.data() + (size_t)ci.output_scanline * ci.output_width * ci.output_components;
+        J->read_lines(&ci, &row, 1);
+    }
+    J->finish(&ci);
+    w = ci.output_width;
+    h = ci.output_height;
+    c = (unsigned)ci.output_components;
+    J->destroy(&ci);
+    il.assign(data.begin(), data.end());
+    return true;
+}
+
 }  // namespace
 
 bool load_frame(const std::string& path, std::vector<float>& planar, unsigned& w, unsigned& h,
@@ -277,7 +492,9 @@ bool load_frame(const std::string& path, std::vector<float>& planar, unsigned& w
     if (e == "tif" || e == "tiff") ok = load_tiff(b, il, w, h, cc, err);
     else if (e == "pfm") ok = load_pfm(b, il, w, h, cc, err);
     else if (e == "pgm" || e == "ppm" || e == "pnm") ok = load_pnm(b, il, w, h, cc, err);
-    else { err = path + ": unsupported image format (use .tif, .pfm, .pgm or .ppm)"; return false; }
+    else if (e == "png") ok = load_png(b, il, w, h, cc, err);
+    else if (e == "jpg" || e == "jpeg") ok = load_jpeg(b, il, w, h, cc, err);
+    else { err = path + ": unsupported image format (use .png, .jpg, .tif, .pfm, .pgm or .ppm)"; return false; }
     if (!ok) { err = path + ": " + err; return false; }
     to_planar(il, w, h, cc, planar, c);
     return true;
@@ -294,7 +511,8 @@ bool save_frame(const std::string& path, const std::vector<float>& planar, unsig
     if (e == "tif" || e == "tiff") return save_tiff(path, il, w, h, c, err);
     if (e == "pfm") return save_pfm(path, il, w, h, c, err);
     if (e == "pgm" || e == "ppm" || e == "pnm") return save_pnm(path, il, w, h, c, err);
-    err = path + ": unsupported image format (use .tif, .pfm, .pgm or .ppm)";
+    if (e == "png") return save_png(path, il, w, h, c, err);
+    err = path + ": unsupported output format (use .png, .tif, .pfm, .pgm or .ppm)";
     return false;
 }
 
@@ -353,3 +571,45 @@ bool save_flow(const std::string& path, const std::vector<float>& planar, unsign
     std::fclose(f);
     return true;
 }
+
+// C entry points of the frame and flow files, for callers outside C++
+// (the command line's tests among them).  0 on success; -1 with the
+// message in vbm3d_io_last_error() otherwise.  A load with out == NULL
+// only reports the sizes.
+namespace {
+thread_local std::string g_io_err;
+}
+
+extern "C" {
+
+const char* vbm3d_io_last_error(void) { return g_io_err.c_str(); }
+
+int vbm3d_io_load_frame(const char* path, float* out, size_t cap, unsigned* w, unsigned* h, unsigned* c)
+{
+    std::vector<float> v;
+    if (!load_frame(path, v, *w, *h, *c, g_io_err)) return -1;
+    if (out) {
+        if (cap < v.size()) { g_io_err = "buffer too small"; return -1; }
+        std::memcpy(out, v.data(), v.size() * sizeof(float));
+    }
+    return 0;
+}
+
+int vbm3d_io_save_frame(const char* path, const float* planar, unsigned w, unsigned h, unsigned c)
+{
+    std::vector<float> v(planar, planar + (size_t)w * h * c);
+    return save_frame(path, v, w, h, c, g_io_err) ? 0 : -1;
+}
+
+int vbm3d_io_load_flow(const char* path, float* out, size_t cap, unsigned* w, unsigned* h)
+{
+    std::vector<float> v;
+    if (!load_flow(path, v, *w, *h, g_io_err)) return -1;
+    if (out) {
+        if (cap < v.size()) { g_io_err = "buffer too small"; return -1; }
+        std::memcpy(out, v.data(), v.size() * sizeof(float));
+    }
+    return 0;
+}
+
+}  // extern "C"

@@ -1,9 +1,10 @@
 /*
  * image_io.h -- frame and flow files for the VBM3Ddenoising command line.
  *
- * Formats, chosen by extension: .tif/.tiff (uncompressed, 8/16-bit
- * integer or 32-bit float samples, chunky), .pfm, .pgm/.ppm (8/16-bit),
- * and .flo (Middlebury optical flow).  Frames are returned planar
+ * Formats, chosen by extension: .png (8/16-bit, read and write; 8-bit
+ * written), .jpg/.jpeg (read), .tif/.tiff (uncompressed, 8/16-bit integer
+ * or 32-bit float samples, chunky), .pfm, .pgm/.ppm (8/16-bit), and .flo
+ * (Middlebury optical flow).  Frames are returned planar
  * [c][y][x] like the reference's loadImage (src/Utilities/LibImages.cpp:
  * 47-110), including its rule that three equal channels collapse to one
  * and a fourth (alpha) channel is dropped.
@@ -24,5 +25,16 @@ bool load_flow(const std::string& path, std::vector<float>& planar, unsigned& w,
                std::string& err);
 bool save_flow(const std::string& path, const std::vector<float>& planar, unsigned w, unsigned h,
                std::string& err);
+
+// The same through a C ABI (libvbm3d_host.so): 0 on success, -1 with the
+// message in vbm3d_io_last_error().  out == NULL only reports the sizes;
+// cap is out's capacity in floats.  Frames are planar [c][y][x], flows
+// [2][y][x].
+extern "C" {
+const char* vbm3d_io_last_error(void);
+int vbm3d_io_load_frame(const char* path, float* out, size_t cap, unsigned* w, unsigned* h, unsigned* c);
+int vbm3d_io_save_frame(const char* path, const float* planar, unsigned w, unsigned h, unsigned c);
+int vbm3d_io_load_flow(const char* path, float* out, size_t cap, unsigned* w, unsigned* h);
+}
 
 #endif

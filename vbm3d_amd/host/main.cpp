@@ -11,8 +11,8 @@
  * (so "-tauHard 2.5" gives 2), float ones accept "a/b", bool ones are
  * false for "false", "off" or "0" (any case).
  *
- * Image formats: .tif/.tiff, .pfm, .pgm/.ppm; flows: .flo (or 2-channel
- * .tif/.pfm).  PNG and JPEG need libraries this build does not link.
+ * Image formats: .png (read and write), .jpg/.jpeg (read), .tif/.tiff,
+ * .pfm, .pgm/.ppm; flows: .flo (or 2-channel .tif/.pfm).
  */
 #include <cctype>
 #include <cmath>

@@ -7,35 +7,35 @@
 
 struct Parameters
 {
-	/// Size of the patch (spatial)
+	/// side of the square patch, in pixels
 	unsigned k;
-	/// Size of the patch (temporal)
+	/// number of frames a patch spans (temporal extent)
 	unsigned kt;
-	/// Step size
+	/// spacing of the reference-patch grid, in pixels
 	unsigned p;
-	/// Maximum number of similar patches
+	/// largest group: similar patches kept per reference patch
 	unsigned N;
-	/// Maximum distance threshold
+	/// distance above which a candidate leaves the group
 	float tau;
-	/// Correcting parameter in the distance computation
+	/// bonus subtracted from the distance at the reference position
 	float d;
-	/// Number of frames forward (and backward) used during the search
+	/// frames searched after (and before) the reference frame
 	unsigned Nf;
-	/// Size of the search region in the reference frame
+	/// side of the search window in the reference frame
 	unsigned Ns;
-	/// Size of the search region in the other frame
+	/// side of the predictive search windows in the other frames
 	unsigned Npr;
-	/// Maximum number of matches kept for a frame
+	/// matches carried from one frame to the next
 	unsigned Nb;
-	/// Type of the 2D tranform
+	/// code of the 2D transform of a patch (DCT or BIOR)
 	unsigned T_2D;
-	/// Type of the 1D tranform
+	/// code of the 1D transform along the group (HADAMARD or HAAR)
 	unsigned T_3D;
-	/// Threshold if it's a hard thresholding step
+	/// hard threshold factor (only read by the hard-threshold step)
 	float lambda3D = 0.;
-	/// Motion compensation for 3d patches
+	/// patches follow the optical flow across their kt frames
 	bool mc = false;
-	/// Border of the tile when using the multithreading
+	/// border of the spatial tiles of a multi-threaded run
 	int n = 16;
 };
 
