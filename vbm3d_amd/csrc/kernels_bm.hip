@@ -274,7 +274,7 @@ __device__ __forceinline__ void bm_one(const A& a, const long r, float* base, co
 #pragma unroll
                                 for (int c = 0; c < C; c++)
                                     lds_dma((const char*)(vid + fb + c * wh) + goff[k],
-                                            tl + (unsigned)(((wn * C + c) * cs1 + k * 64) * 4));
+                                            tl + (unsigned)((wn * L.ws + c * cs1 + k * 64) * 4));
                             }
                         }
                     }
@@ -302,7 +302,7 @@ __device__ __forceinline__ void bm_one(const A& a, const long r, float* base, co
                 const bool valid = act[h] & (sm < Nb) & (sl < P2) & (slx < wx) & (sly < wy) & !seen;
                 if (valid) {
                     const int wn = h * Nb + sm;
-                    const unsigned toff = (unsigned)(wn * C * cs1 + sly * L.tp1 + slx);
+                    const unsigned toff = (unsigned)(wn * L.ws + sly * L.tp1 + slx);
                     const bool self = (unsigned)qx == rx && (unsigned)qy == ry;
                     s_d[h * 64 + lane] = __uint_as_float(toff | (self ? 0x80000000u : 0u));
                     s_i[h * 64 + lane] = (unsigned)(m1 ? wb[4] : wa[4]) * whc + qy * W + qx;
@@ -417,7 +417,7 @@ __device__ __forceinline__ void bm_one(const A& a, const long r, float* base, co
         wave_sync();
         for (int wn = 0; wn < 2 * Nb; wn++)
             if (act[wn / Nb])
-                load_tile(tile + wn * C * cs1, L.tp1, cs1, win[wn * 5], win[wn * 5 + 1], win[wn * 5 + 2],
+                load_tile(tile + wn * L.ws, L.tp1, cs1, win[wn * 5], win[wn * 5 + 1], win[wn * 5 + 2],
                           win[wn * 5 + 3], (unsigned)win[wn * 5 + 4]);
         wave_sync();
         // slots: chain ch, window m, candidate l -> (ch * Nb + m) * P2p + l;
@@ -444,7 +444,7 @@ __device__ __forceinline__ void bm_one(const A& a, const long r, float* base, co
                                 seen |= qx >= w2[0] && qx < w2[0] + w2[2] && qy >= w2[1] && qy < w2[1] + w2[3];
                             }
                             valid[h] = !seen;
-                            toff[h] = wn * C * cs1 + ly * L.tp1 + lx;
+                            toff[h] = wn * L.ws + ly * L.tp1 + lx;
                             qxs[h] = qx;
                             qys[h] = qy;
                             tfs[h] = (unsigned)win[wn * 5 + 4];

@@ -537,8 +537,9 @@ __global__ __launch_bounds__(64) void k_filter_gen(GenArgs a) {
 template <int C>
 __global__ __launch_bounds__(kAggThreads) void k_aggregate_gen(AArgs a, const uint32_t* __restrict__ mpos,
                                                                const TabsG* __restrict__ tg) {
-    constexpr int TX = kAggTileX, TY = kAggTileY, NT = kAggThreads, NS = kGenAggScan;
+    constexpr int TX = kGenAggTileX, TY = kGenAggTileY, NT = kAggThreads, NS = kGenAggScan;
     constexpr int QS = 8, NWV = NT / 64, WX = TX / QS;
+    static_assert(TX * TY == NT && (TX / QS) * (TY / QS) == NWV, "one thread per pixel, 8x8 per wavefront");
     // the ordered list of one chunk of NS scanned reference patches, at
     // most N entries each: slot, member | slice << 8, position
     extern __shared__ uint32_t agg_lists[];
@@ -718,7 +719,7 @@ hipError_t launch_aggregate_gen(int C, const AArgs& a, const uint32_t* mpos, con
                                 hipStream_t s) {
     if (frames == 0) return hipSuccess;
     if (a.N < 1 || a.N > kGenMaxN) return hipErrorInvalidValue;
-    dim3 grid((a.W + kAggTileX - 1) / kAggTileX, (a.H + kAggTileY - 1) / kAggTileY, frames);
+    dim3 grid((a.W + kGenAggTileX - 1) / kGenAggTileX, (a.H + kGenAggTileY - 1) / kGenAggTileY, frames);
     const size_t lds = (size_t)3 * kGenAggScan * a.N * sizeof(uint32_t);
     if (C == 1) hipLaunchKernelGGL((k_aggregate_gen<1>), grid, dim3(kAggThreads), lds, s, a, mpos, tabs);
     else hipLaunchKernelGGL((k_aggregate_gen<3>), grid, dim3(kAggThreads), lds, s, a, mpos, tabs);

@@ -14,12 +14,19 @@ Dispatches are grouped by kernel name; a kernel that runs in both steps of
 a run (``--alternate NAME``, e.g. the aggregation, whose template is the
 same for both steps) is split by dispatch order into its step-1 and step-2
 launches, since every run launches it once per step, step 1 first.
+``--runs R`` keeps each kernel's dispatches of the first R runs only
+(bench.py: the warm-up and timed device-buffer runs, which precede its
+host-buffer leg, whose chunked launches are smaller).
 
 FETCH_SIZE and WRITE_SIZE are KiB per dispatch as the TCC sees them.  Each
 is scaled by the factor that maps the largest plain copy in the same run
 (torch's ``copy_`` of the pristine clip into the work buffer, a copyBuffer
 kernel that reads and writes exactly ``--copy-bytes``) to its known size,
 so the figures are HBM bytes per launch.
+
+GRBM_GUI_ACTIVE (the launch's busy cycles) is reported summed over the 8
+XCDs, so it is divided by 8 here (rocprof's own formulas take the maximum
+over XCDs, which equals the mean for launches that fill every XCD).
 
 LDS figures: LDS busy = SQ_LDS_IDX_ACTIVE / (GRBM_GUI_ACTIVE * CUs), the
 fraction of the launch's cycles the CUs' LDS served indexed accesses (its
@@ -51,6 +58,7 @@ import os
 import re
 
 CUS = 256
+XCDS = 8  # GRBM_GUI_ACTIVE arrives summed over the XCDs; its per-XCD mean stands in for rocprof's max
 LDS_PEAK_TBS = 128 * CUS * 2.4e9 / 1e12
 
 
@@ -59,17 +67,24 @@ def short_name(name: str) -> str:
     return n.split("(")[0].strip()
 
 
+RUNS = None  # keep only each kernel's dispatches of the first RUNS runs
+
+
 def label(names_in_order, alternate):
     """Per dispatch (in dispatch order): the kernel name, with ' (step 1)'
-    / ' (step 2)' appended alternately for the kernels in `alternate`."""
+    / ' (step 2)' appended alternately for the kernels in `alternate`, or
+    None for a dispatch past the first RUNS runs of its kernel."""
     seen = collections.Counter()
     out = []
     for n in names_in_order:
-        if n in alternate:
+        per_run = 2 if n in alternate else 1
+        if RUNS is not None and "copyBuffer" not in n and seen[n] >= RUNS * per_run:
+            out.append(None)
+        elif n in alternate:
             out.append(f"{n} (step {1 + seen[n] % 2})")
-            seen[n] += 1
         else:
             out.append(n)
+        seen[n] += 1
     return out
 
 
@@ -83,6 +98,8 @@ def read_trace(path, alternate):
     labels = label([k for _, k, _ in rows], alternate)
     out = collections.OrderedDict()
     for (_, _, ms), lab in zip(rows, labels):
+        if lab is None:
+            continue
         e = out.setdefault(lab, [0, 0.0])
         e[0] += 1
         e[1] += ms
@@ -112,6 +129,8 @@ def read_counters(path, alternate):
     labels = label([names[d] for d in order], alternate)
     out = collections.defaultdict(lambda: collections.defaultdict(list))
     for d, lab in zip(order, labels):
+        if lab is None:
+            continue
         for c, v in per[d].items():
             out[lab][c].append(v)
     return out
@@ -134,10 +153,14 @@ def main():
     ap.add_argument("--md", required=True)
     ap.add_argument("--json")
     ap.add_argument("--alternate", nargs="*", default=[])
+    ap.add_argument("--runs", type=int, help="only the first RUNS runs (dispatches of each kernel in order); "
+                                             "bench.py's device-buffer runs come before its host-buffer leg")
     ap.add_argument("--commands", nargs="*", default=[])
     ap.add_argument("--min-ms", type=float, default=0.5, help="list kernels whose average (total, without counters) reaches this")
     args = ap.parse_args()
     alt = set(args.alternate)
+    global RUNS
+    RUNS = args.runs
 
     stats = read_trace(args.trace, alt) if args.trace else read_stats(args.stats)
     if not (args.fetch and args.write):
@@ -195,7 +218,7 @@ def main():
                 continue
             c = ins[k]
             waves = mean(c.get("SQ_WAVES", []))
-            gui = mean(c.get("GRBM_GUI_ACTIVE", []))
+            gui = mean(c.get("GRBM_GUI_ACTIVE", [])) / XCDS
             act = mean(c.get("SQ_ACTIVE_INST_VALU", []))
             thr = mean(c.get("SQ_THREAD_CYCLES_VALU", []))
             lds = mean(c.get("SQ_LDS_IDX_ACTIVE", []))
