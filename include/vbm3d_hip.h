@@ -248,6 +248,43 @@ int vbm3d_profile(vbm3d_ctx* ctx, int enable);
 int vbm3d_profile_read(vbm3d_ctx* ctx, int kernel, double* ms, long* launches,
                        double* work);
 
+/* ---- TV-L1 optical flow (the flow companion the reference ships for
+ * -fflow / -bflow: tvl1flow/tvl1flow_lib.c, driven by tvl1flow/main.c
+ * and src/utils/VBM3Ddenoising_OF.sh).  Images are grey float [ny][nx];
+ * the flow is returned as u (x component) and v (y component), each
+ * [ny][nx], the planes the reference writes to its flow file. */
+typedef struct vbm3d_tvl1_params {
+    float tau;      /* time step of the dual update */
+    float lambda;   /* weight of the data term */
+    float theta;    /* coupling of the primal and auxiliary flows */
+    int nscales;    /* pyramid levels (capped by the caller: vbm3d_tvl1_nscales) */
+    int fscale;     /* finest level solved; finer levels are only upsampled */
+    float zfactor;  /* downsampling factor between levels, in (0, 1) */
+    int nwarps;     /* warps per level */
+    float epsilon;  /* stopping threshold on the RMS flow change */
+} vbm3d_tvl1_params;
+
+/* The command line's defaults (tvl1flow/main.c:20-30). */
+void vbm3d_tvl1_default_params(vbm3d_tvl1_params* p);
+
+/* The level count the reference's command line uses: `nscales`, capped so
+ * that the coarsest level stays about 16 pixels across (main.c:139-146). */
+int vbm3d_tvl1_nscales(int nx, int ny, float zfactor, int nscales);
+
+/* Dual_TVL1_optic_flow_multiscale (tvl1flow_lib.c:384-485) on the GPU:
+ * normalisation, pre-smoothing, pyramid, and per level `nwarps` warps of
+ * at most 300 primal-dual iterations.  iters (optional) receives the
+ * iterations of each solved level (coarsest first) and warp:
+ * (nscales - fscale) * nwarps ints.  The _device form takes device
+ * buffers and runs on the context's stream; the plain form copies host
+ * buffers.  Both return VBM3D_OK or an error code. */
+int vbm3d_tvl1_flow(vbm3d_ctx* ctx, const float* I0, const float* I1, float* u,
+                    float* v, int nx, int ny, const vbm3d_tvl1_params* p,
+                    int* iters);
+int vbm3d_tvl1_flow_device(vbm3d_ctx* ctx, const float* d_I0, const float* d_I1,
+                           float* d_u, float* d_v, int nx, int ny,
+                           const vbm3d_tvl1_params* p, int* iters);
+
 /* Host-side check of the group-selection routine the kernels use: sorts
  * the first `mid` of `len` (dist, idx) pairs like std::partial_sort with
  * the reference comparator (src/Utilities/Utilities.cpp:40-45). */

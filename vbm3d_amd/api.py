@@ -475,3 +475,88 @@ def profile_read(kernel: str, ctx=None, step: int = 0):
     code = KERNELS[kernel] + 8 * step
     _check(lib().vbm3d_profile_read(ctx.h, code, C.byref(ms), C.byref(n), C.byref(w)))
     return ms.value, n.value, w.value
+
+
+class TVL1Params(C.Structure):
+    """POD mirror of vbm3d_tvl1_params: the TV-L1 flow's settings, in the
+    order tvl1flow/main.c reads them from its command line."""
+    _fields_ = [("tau", C.c_float), ("lambda_", C.c_float), ("theta", C.c_float), ("nscales", C.c_int),
+                ("fscale", C.c_int), ("zfactor", C.c_float), ("nwarps", C.c_int), ("epsilon", C.c_float)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+def _bind_tvl1(L):
+    if getattr(L, "_tvl1_bound", False):
+        return L
+    fp, tp, ip = C.POINTER(C.c_float), C.POINTER(TVL1Params), C.POINTER(C.c_int)
+    L.vbm3d_tvl1_default_params.argtypes = [tp]
+    L.vbm3d_tvl1_nscales.argtypes = [C.c_int, C.c_int, C.c_float, C.c_int]
+    L.vbm3d_tvl1_flow.argtypes = [C.c_void_p, fp, fp, fp, fp, C.c_int, C.c_int, tp, ip]
+    L.vbm3d_tvl1_flow_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
+                                         C.c_int, tp, ip]
+    L._tvl1_bound = True
+    return L
+
+
+def tvl1_params(tau=None, lam=None, theta=None, nscales=None, fscale=None, zfactor=None, nwarps=None,
+                epsilon=None) -> TVL1Params:
+    """The flow command line's defaults (tvl1flow/main.c:20-30), with any
+    value given here replacing its default."""
+    p = TVL1Params()
+    _bind_tvl1(lib()).vbm3d_tvl1_default_params(C.byref(p))
+    for name, val in (("tau", tau), ("lambda_", lam), ("theta", theta), ("nscales", nscales), ("fscale", fscale),
+                      ("zfactor", zfactor), ("nwarps", nwarps), ("epsilon", epsilon)):
+        if val is not None:
+            setattr(p, name, val)
+    return p
+
+
+def tvl1_nscales(nx, ny, zfactor=0.5, nscales=100) -> int:
+    """Pyramid levels as the reference's command line caps them
+    (tvl1flow/main.c:139-146)."""
+    return _bind_tvl1(lib()).vbm3d_tvl1_nscales(nx, ny, zfactor, nscales)
+
+
+def tvl1_flow(I0, I1, params: TVL1Params | None = None, cap_scales=True, ctx=None):
+    """TV-L1 optical flow from grey image I0 to I1 ([ny][nx] float32) on
+    the GPU (Dual_TVL1_optic_flow_multiscale, tvl1flow/tvl1flow_lib.c:
+    384-485).  cap_scales applies the command line's level cap.  Returns
+    (u, v, iterations [solved level, coarsest first][warp])."""
+    L = _bind_tvl1(lib())
+    I0 = np.ascontiguousarray(I0, np.float32)
+    I1 = np.ascontiguousarray(I1, np.float32)
+    if I0.ndim != 2 or I0.shape != I1.shape:
+        raise ValueError(f"I0 and I1 must be equal 2-D arrays, got {I0.shape} and {I1.shape}")
+    ny, nx = I0.shape
+    p = tvl1_params() if params is None else params
+    if cap_scales:
+        p = TVL1Params(**{f: getattr(p, f) for f, _ in TVL1Params._fields_})
+        p.nscales = L.vbm3d_tvl1_nscales(nx, ny, p.zfactor, p.nscales)
+        p.fscale = min(p.fscale, p.nscales)
+    u = np.zeros((ny, nx), np.float32)
+    v = np.zeros((ny, nx), np.float32)
+    nl = max(p.nscales - p.fscale, 0)
+    iters = np.zeros(max(nl * p.nwarps, 1), np.int32)
+    c = ctx or default_context()
+    _check(L.vbm3d_tvl1_flow(c.h, _f(I0), _f(I1), _f(u), _f(v), nx, ny, C.byref(p),
+                             iters.ctypes.data_as(C.POINTER(C.c_int))))
+    return u, v, iters[: nl * p.nwarps].reshape(nl, p.nwarps)
+
+
+def tvl1_flow_device(I0_t, I1_t, u_t, v_t, params: TVL1Params | None = None, ctx=None):
+    """The same on device tensors ([ny][nx] float32, contiguous); u_t and
+    v_t receive the flow.  Returns the iteration counts."""
+    L = _bind_tvl1(lib())
+    ny, nx = I0_t.shape
+    for t in (I1_t, u_t, v_t):
+        if tuple(t.shape) != (ny, nx):
+            raise ValueError("all four tensors must have I0's shape")
+    p = tvl1_params() if params is None else params
+    nl = max(p.nscales - p.fscale, 0)
+    iters = np.zeros(max(nl * p.nwarps, 1), np.int32)
+    c = ctx or default_context()
+    _check(L.vbm3d_tvl1_flow_device(c.h, _ptr(I0_t), _ptr(I1_t), _ptr(u_t), _ptr(v_t), nx, ny, C.byref(p),
+                                    iters.ctypes.data_as(C.POINTER(C.c_int))))
+    return iters[: nl * p.nwarps].reshape(nl, p.nwarps)

@@ -20,7 +20,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libvbm3d_hip.so")
 # one translation unit per kernel family, compiled in parallel
 SOURCES = [os.path.join(CSRC, f) for f in ("kernels_bm.hip", "kernels_filter.hip", "kernels_agg.hip",
-                                           "kernels_gen.hip", "capi.hip")]
+                                           "kernels_gen.hip", "flow.hip", "capi.hip")]
 OBJDIR = os.path.join(HERE, "_build")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("VBM3D_ARCH", "gfx950")
@@ -37,6 +37,7 @@ FLAGS = [
 HOST = os.path.join(HERE, "host")
 HOST_LIB = os.path.join(HERE, "libvbm3d_host.so")
 CLI = os.path.join(HERE, "bin", "VBM3Ddenoising")
+FLOW_CLI = os.path.join(os.path.dirname(CLI), "tvl1flow")
 CXX = os.environ.get("CXX", "g++")
 CXXFLAGS = ["-O2", "-std=c++17", "-Wall", "-Wextra", f"-I{os.path.join(ROOT, 'include')}"]
 IMG_INC = os.environ.get("VBM3D_IMAGE_INCLUDE", "/opt/conda/include")  # png.h, jpeglib.h
@@ -78,6 +79,10 @@ def build(verbose: bool = False) -> str:
     _run([CXX, *CXXFLAGS, "-o", CLI + ".tmp", os.path.join(HOST, "main.cpp"),
           f"-L{HERE}", "-l:libvbm3d_host.so", "-l:libvbm3d_hip.so", "-Wl,-rpath,$ORIGIN/.."], verbose)
     os.replace(CLI + ".tmp", CLI)
+    # the TV-L1 flow command line (the reference's tvl1flow program)
+    _run([CXX, *CXXFLAGS, "-o", FLOW_CLI + ".tmp", os.path.join(HOST, "tvl1flow_main.cpp"),
+          f"-L{HERE}", "-l:libvbm3d_host.so", "-l:libvbm3d_hip.so", "-Wl,-rpath,$ORIGIN/.."], verbose)
+    os.replace(FLOW_CLI + ".tmp", FLOW_CLI)
     return LIB
 
 

@@ -357,6 +357,7 @@ struct vbm3d_ctx {
     hipEvent_t ready[3] = {nullptr, nullptr, nullptr};  // noisy final, basic final, upload chunk
     DevBuf idx, nsx, nsim, dist, est, wgt, tabs, refxy, meta, gpos, rlist, scal, stats, counts;
     DevBuf tabsg, mpos;  // general path: tables, members' slice corners
+    DevBuf tv;           // TV-L1 flow workspace (flow.hip)
     size_t est_budget = 0;  // bytes for the per-patch rings; 0 = from free device memory
     unsigned chunk_frames = 0;  // per-clip path: frames per chunk; 0 = from the budget
     // kernel profiling (vbm3d_profile / vbm3d_profile_read)
@@ -1255,6 +1256,22 @@ extern "C" {
 
 const char* vbm3d_last_error(void) { return g_err.c_str(); }
 
+}  // extern "C"
+
+// Context access for the flow entry points (flow.hip).
+namespace vb {
+hipStream_t ctx_stream(vbm3d_ctx* c) { return c->stream; }
+int ctx_device(vbm3d_ctx* c) { return c->device; }
+hipError_t ctx_flow_workspace(vbm3d_ctx* c, size_t bytes, void** p) {
+    const hipError_t e = ensure(c->tv, bytes);
+    *p = c->tv.p;
+    return e;
+}
+int ctx_fail(int code, const std::string& msg) { return fail(code, msg); }
+}  // namespace vb
+
+extern "C" {
+
 vbm3d_ctx* vbm3d_create(int device) {
     if (hipSetDevice(device) != hipSuccess) {
         g_err = "hipSetDevice failed";
@@ -1280,7 +1297,7 @@ void vbm3d_destroy(vbm3d_ctx* c) {
                       &c->idx, &c->nsx, &c->nsim, &c->dist, &c->est, &c->wgt, &c->tabs,
                       &c->refxy, &c->meta, &c->gpos, &c->rlist, &c->scal, &c->stats, &c->counts,
                       &c->b_idx, &c->b_nsx, &c->b_est, &c->b_wgt, &c->b_refxy, &c->b_meta, &c->b_gpos,
-                      &c->b_tabs, &c->b_plan, &c->b_cnt, &c->b_save})
+                      &c->b_tabs, &c->b_plan, &c->b_cnt, &c->b_save, &c->tabsg, &c->mpos, &c->tv})
         if (b->p) (void)hipFree(b->p);
     if (c->b_exec) (void)hipGraphExecDestroy(c->b_exec);
     if (c->b_graph) (void)hipGraphDestroy(c->b_graph);

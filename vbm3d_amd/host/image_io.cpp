@@ -480,15 +480,17 @@ bool load_jpeg(const std::vector<uint8_t>& b, std::vector<float>& il, unsigned& 
 
 }  // namespace
 
-bool load_frame(const std::string& path, std::vector<float>& planar, unsigned& w, unsigned& h,
-                unsigned& c, std::string& err)
+// Interleaved samples of any supported file; `bytes` is true for the
+// formats whose samples are 8-bit integers (PNG, JPEG and PNM at 8 bits,
+// TIFF whose samples are all integers in [0, 255]).
+bool load_interleaved(const std::string& path, std::vector<float>& il, unsigned& w, unsigned& h, unsigned& cc,
+                      bool& bytes, std::string& err)
 {
     std::vector<uint8_t> b;
     if (!read_all(path, b, err)) return false;
-    std::vector<float> il;
-    unsigned cc = 0;
     const std::string e = ext_of(path);
     bool ok;
+    bytes = false;
     if (e == "tif" || e == "tiff") ok = load_tiff(b, il, w, h, cc, err);
     else if (e == "pfm") ok = load_pfm(b, il, w, h, cc, err);
     else if (e == "pgm" || e == "ppm" || e == "pnm") ok = load_pnm(b, il, w, h, cc, err);
@@ -496,7 +498,43 @@ bool load_frame(const std::string& path, std::vector<float>& planar, unsigned& w
     else if (e == "jpg" || e == "jpeg") ok = load_jpeg(b, il, w, h, cc, err);
     else { err = path + ": unsupported image format (use .png, .jpg, .tif, .pfm, .pgm or .ppm)"; return false; }
     if (!ok) { err = path + ": " + err; return false; }
+    if (e != "pfm") {
+        bytes = true;
+        for (float v : il)
+            if (!(v >= 0.f && v <= 255.f && v == std::floor(v))) { bytes = false; break; }
+    }
+    return true;
+}
+
+bool load_frame(const std::string& path, std::vector<float>& planar, unsigned& w, unsigned& h,
+                unsigned& c, std::string& err)
+{
+    std::vector<float> il;
+    unsigned cc = 0;
+    bool bytes;
+    if (!load_interleaved(path, il, w, h, cc, bytes, err)) return false;
     to_planar(il, w, h, cc, planar, c);
+    return true;
+}
+
+bool load_grey(const std::string& path, std::vector<float>& img, unsigned& w, unsigned& h, std::string& err)
+{
+    std::vector<float> il;
+    unsigned cc = 0;
+    bool bytes;
+    if (!load_interleaved(path, il, w, h, cc, bytes, err)) return false;
+    const size_t n = (size_t)w * h;
+    img.resize(n);
+    if (cc == 1) {
+        std::memcpy(img.data(), il.data(), n * sizeof(float));
+        return true;
+    }
+    if (cc != 3 && cc != 4) { err = path + ": non-scalarizable image"; return false; }
+    for (size_t i = 0; i < n; i++) {
+        const float* q = &il[i * cc];
+        const double g = .299 * q[0] + .587 * q[1] + .114 * q[2];
+        img[i] = bytes ? (float)(uint8_t)g : (float)g;
+    }
     return true;
 }
 

@@ -20,6 +20,11 @@ bool load_frame(const std::string& path, std::vector<float>& planar, unsigned& w
                 unsigned& c, std::string& err);
 bool save_frame(const std::string& path, const std::vector<float>& planar, unsigned w, unsigned h,
                 unsigned c, std::string& err);
+// One grey plane [y][x] as the flow program reads its images
+// (tvl1flow/iio.c:2683-2702, 808-871): RGB and RGBA become
+// .299 R + .587 G + .114 B in double, truncated to an integer for 8-bit
+// files; grey + alpha is refused.
+bool load_grey(const std::string& path, std::vector<float>& img, unsigned& w, unsigned& h, std::string& err);
 // Two-channel flow, planar [2][y][x] (u then v).
 bool load_flow(const std::string& path, std::vector<float>& planar, unsigned& w, unsigned& h,
                std::string& err);
