@@ -285,6 +285,59 @@ int vbm3d_tvl1_flow_device(vbm3d_ctx* ctx, const float* d_I0, const float* d_I1,
                            float* d_u, float* d_v, int nx, int ny,
                            const vbm3d_tvl1_params* p, int* iters);
 
+/* ---- Multiscale pyramid (multiscaler/: decompose.cpp, recompose.cpp,
+ * merge_coarse.cpp, the functions of multiscaler.cpp) and the multiscale
+ * denoising it serves (src/utils/VBM3Ddenoising_multiscale.sh).  Images
+ * are planar float [c][rows][cols] host buffers, c = 1 or 3; a pyramid of
+ * `levels` levels is stored level after level, level i of
+ * rows_i x cols_i, each half the previous (integer division).  type: 0 the
+ * DCT pyramid (full-frame DCT, low frequencies kept), 1 Gaussian,
+ * 2 Lanczos; factor is the recomposition ratio (-c). */
+
+/* Level sizes (decompose.cpp:44-53). */
+int vbm3d_pyr_levels(int rows, int cols, int levels, int* lrows, int* lcols);
+
+/* decompose: the levels of one image. */
+int vbm3d_pyr_decompose(vbm3d_ctx* ctx, const float* img, int rows, int cols,
+                        int ch, int levels, int type, float* out);
+
+/* recompose: one image from its levels; type 0 copies the rows_i * factor
+ * x cols_i * factor lowest DCT frequencies of every coarser level into the
+ * finest level's, types 1 and 2 add back blurred, upsampled differences. */
+int vbm3d_pyr_recompose(vbm3d_ctx* ctx, const float* levels, int rows,
+                        int cols, int ch, int nlevels, int type, float factor,
+                        float* out);
+
+/* merge_coarse: the coarse image's lowest DCT frequencies replace the
+ * fine image's. */
+int vbm3d_pyr_merge_coarse(vbm3d_ctx* ctx, const float* fine, int rows,
+                           int cols, const float* coarse, int crows, int ccols,
+                           int ch, float factor, float* out);
+
+/* The multiscale denoising of a video [F][C][H][W]: every frame into its
+ * levels, every level denoised as a video by run_vbm3d with sigma / 2^level
+ * truncated to two decimals (the driver script's `bc`) and default
+ * parameters for that sigma (or p1 / p2 when not NULL), every frame
+ * recomposed.  noisy_levels / denoised_levels (optional) receive the
+ * levels, level after level, each [F][C][rows_i][cols_i]. */
+int vbm3d_multiscale_denoise(vbm3d_ctx* ctx, float sigma, const float* noisy,
+                             unsigned W, unsigned H, unsigned F, unsigned C,
+                             const vbm3d_params* p1, const vbm3d_params* p2,
+                             int levels, int type, float factor,
+                             int color_space, int ref_grid_nb, float* denoised,
+                             float* noisy_levels, float* denoised_levels);
+
+/* Micro-benchmark of the group filter's k x k DCT (k = 7 or 8), with no
+ * counterpart in the reference: `reps` forward + inverse round trips over
+ * `npatch` random patches kept on chip, on the vector ALUs with the
+ * denoiser's own tables and operation order (mode 0) or on the matrix
+ * cores, FP32 MFMA with the orthonormal DCT matrix (mode 1), on the
+ * current device.  Returns the milliseconds of one launch (median of 3
+ * after a warm-up) and the largest |round trip - input| after one round
+ * trip.  It decides whether MFMA pays for these transforms. */
+int vbm3d_bench_transforms(int K, int mode, long npatch, int reps, double* ms,
+                           double* max_err);
+
 /* Host-side check of the group-selection routine the kernels use: sorts
  * the first `mid` of `len` (dist, idx) pairs like std::partial_sort with
  * the reference comparator (src/Utilities/Utilities.cpp:40-45). */

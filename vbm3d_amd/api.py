@@ -560,3 +560,129 @@ def tvl1_flow_device(I0_t, I1_t, u_t, v_t, params: TVL1Params | None = None, ctx
     _check(L.vbm3d_tvl1_flow_device(c.h, _ptr(I0_t), _ptr(I1_t), _ptr(u_t), _ptr(v_t), nx, ny, C.byref(p),
                                     iters.ctypes.data_as(C.POINTER(C.c_int))))
     return iters[: nl * p.nwarps].reshape(nl, p.nwarps)
+
+
+def bench_transforms(K=7, mode=0, npatch=1 << 22, reps=32):
+    """The transform micro-benchmark (vbm3d_bench_transforms): k x k DCT
+    round trips on the VALU (mode 0, the denoiser's exact transforms) or
+    the matrix cores (mode 1, FP32 MFMA).  Returns (ms per launch, 2D
+    transforms per second, largest round-trip error)."""
+    L = lib()
+    if not getattr(L, "_xb_bound", False):
+        L.vbm3d_bench_transforms.argtypes = [C.c_int, C.c_int, C.c_long, C.c_int, C.POINTER(C.c_double),
+                                             C.POINTER(C.c_double)]
+        L._xb_bound = True
+    ms, err = C.c_double(), C.c_double()
+    _check(L.vbm3d_bench_transforms(K, mode, npatch, reps, C.byref(ms), C.byref(err)))
+    return ms.value, 2.0 * npatch * reps / (ms.value * 1e-3), err.value
+
+
+# ---- multiscale pyramid (multiscaler/) and multiscale denoising
+PYR_DCT, PYR_GAUSSIAN, PYR_LANCZOS = 0, 1, 2
+
+
+def _bind_pyr(L):
+    if getattr(L, "_pyr_bound", False):
+        return L
+    fp, ip, pp = C.POINTER(C.c_float), C.POINTER(C.c_int), C.POINTER(Parameters)
+    L.vbm3d_pyr_levels.argtypes = [C.c_int, C.c_int, C.c_int, ip, ip]
+    L.vbm3d_pyr_decompose.argtypes = [C.c_void_p, fp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, fp]
+    L.vbm3d_pyr_recompose.argtypes = [C.c_void_p, fp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float, fp]
+    L.vbm3d_pyr_merge_coarse.argtypes = [C.c_void_p, fp, C.c_int, C.c_int, fp, C.c_int, C.c_int, C.c_int, C.c_float,
+                                         fp]
+    L.vbm3d_multiscale_denoise.argtypes = [C.c_void_p, C.c_float, fp, C.c_uint, C.c_uint, C.c_uint, C.c_uint, pp, pp,
+                                           C.c_int, C.c_int, C.c_float, C.c_int, C.c_int, fp, fp, fp]
+    L._pyr_bound = True
+    return L
+
+
+def _planar(img):
+    a = np.ascontiguousarray(img, np.float32)
+    if a.ndim == 2:
+        a = a[None]
+    if a.ndim != 3:
+        raise ValueError("an image is [c][rows][cols] (or [rows][cols])")
+    return a
+
+
+def pyr_levels(rows, cols, levels):
+    """Level sizes: each level halves the previous (decompose.cpp:44-53)."""
+    L = _bind_pyr(lib())
+    lr = np.zeros(levels, np.int32)
+    lc = np.zeros(levels, np.int32)
+    _check(L.vbm3d_pyr_levels(rows, cols, levels, lr.ctypes.data_as(C.POINTER(C.c_int)),
+                              lc.ctypes.data_as(C.POINTER(C.c_int))))
+    return list(zip(lr.tolist(), lc.tolist()))
+
+
+def pyr_decompose(img, levels, type_=PYR_DCT, ctx=None):
+    """multiscaler's decompose on the GPU: the list of levels of one image."""
+    L = _bind_pyr(lib())
+    a = _planar(img)
+    ch, rows, cols = a.shape
+    sz = pyr_levels(rows, cols, levels)
+    out = np.zeros(sum(ch * r * c for r, c in sz), np.float32)
+    c = ctx or default_context()
+    _check(L.vbm3d_pyr_decompose(c.h, _f(a), rows, cols, ch, levels, type_, _f(out)))
+    res, o = [], 0
+    for r, cc in sz:
+        res.append(out[o:o + ch * r * cc].reshape(ch, r, cc).copy())
+        o += ch * r * cc
+    return res
+
+
+def pyr_recompose(levels, type_=PYR_DCT, factor=0.8, ctx=None):
+    """multiscaler's recompose on the GPU (-c factor, -t type)."""
+    L = _bind_pyr(lib())
+    lev = [_planar(l) for l in levels]
+    ch, rows, cols = lev[0].shape
+    if [tuple(l.shape[1:]) for l in lev] != pyr_levels(rows, cols, len(lev)) or any(l.shape[0] != ch for l in lev):
+        raise ValueError("levels must halve the previous one's size and share the channel count")
+    flat = np.concatenate([l.ravel() for l in lev])
+    out = np.zeros((ch, rows, cols), np.float32)
+    c = ctx or default_context()
+    _check(L.vbm3d_pyr_recompose(c.h, _f(flat), rows, cols, ch, len(lev), type_, factor, _f(out)))
+    return out
+
+
+def pyr_merge_coarse(fine, coarse, factor=0.8, ctx=None):
+    """multiscaler's merge_coarse on the GPU."""
+    L = _bind_pyr(lib())
+    f, g = _planar(fine), _planar(coarse)
+    out = np.zeros_like(f)
+    c = ctx or default_context()
+    _check(L.vbm3d_pyr_merge_coarse(c.h, _f(f), f.shape[1], f.shape[2], _f(g), g.shape[1], g.shape[2], f.shape[0],
+                                    factor, _f(out)))
+    return out
+
+
+def multiscale_denoise(noisy, sigma, levels=3, type_=PYR_DCT, factor=0.7, prms_1=None, prms_2=None,
+                       color_space=True, ref_grid_nb=1, return_levels=False, ctx=None):
+    """The reference's multiscale denoising (src/utils/VBM3Ddenoising_
+    multiscale.sh, defaults: 3 levels, DCT pyramid, recomposition ratio
+    0.7) of a [F][C][H][W] float32 video.  Returns the denoised video, and
+    with return_levels also the noisy and denoised levels (lists of
+    [F][C][rows_i][cols_i])."""
+    L = _bind_pyr(lib())
+    v = _video(np.ascontiguousarray(noisy, np.float32), "noisy")
+    F, Ch, H, W = v.shape
+    out = np.zeros_like(v)
+    sz = pyr_levels(H, W, levels)
+    nl = sum(F * Ch * r * cc for r, cc in sz)
+    nlev = np.zeros(nl, np.float32) if return_levels else None
+    dlev = np.zeros(nl, np.float32) if return_levels else None
+    c = ctx or default_context()
+    _check(L.vbm3d_multiscale_denoise(c.h, float(sigma), _f(v), W, H, F, Ch,
+                                      None if prms_1 is None else C.byref(prms_1),
+                                      None if prms_2 is None else C.byref(prms_2), levels, type_, factor,
+                                      int(bool(color_space)), ref_grid_nb, _f(out), _f(nlev), _f(dlev)))
+    if not return_levels:
+        return out
+
+    def split(flat):
+        res, o = [], 0
+        for r, cc in sz:
+            res.append(flat[o:o + F * Ch * r * cc].reshape(F, Ch, r, cc).copy())
+            o += F * Ch * r * cc
+        return res
+    return out, split(nlev), split(dlev)

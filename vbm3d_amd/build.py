@@ -20,7 +20,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libvbm3d_hip.so")
 # one translation unit per kernel family, compiled in parallel
 SOURCES = [os.path.join(CSRC, f) for f in ("kernels_bm.hip", "kernels_filter.hip", "kernels_agg.hip",
-                                           "kernels_gen.hip", "flow.hip", "capi.hip")]
+                                           "kernels_gen.hip", "flow.hip", "xform_bench.hip", "pyramid.hip", "capi.hip")]
 OBJDIR = os.path.join(HERE, "_build")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("VBM3D_ARCH", "gfx950")
@@ -66,7 +66,9 @@ def build(verbose: bool = False) -> str:
         if p.wait() != 0:
             raise subprocess.CalledProcessError(p.returncode, cmd)
     tmp = LIB + ".tmp"
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-o", tmp, *objs], verbose)
+    # rocBLAS: the pyramid's full-frame DCTs as DGEMMs
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-o", tmp, *objs, "-L/opt/rocm/lib", "-lrocblas",
+          "-Wl,-rpath,/opt/rocm/lib"], verbose)
     os.replace(tmp, LIB)
     host_src = [os.path.join(HOST, "vbm3d_host.cpp"), os.path.join(HOST, "image_io.cpp")]
     # PNG through the system's libpng16 (headers of the same 1.6 release
@@ -83,6 +85,12 @@ def build(verbose: bool = False) -> str:
     _run([CXX, *CXXFLAGS, "-o", FLOW_CLI + ".tmp", os.path.join(HOST, "tvl1flow_main.cpp"),
           f"-L{HERE}", "-l:libvbm3d_host.so", "-l:libvbm3d_hip.so", "-Wl,-rpath,$ORIGIN/.."], verbose)
     os.replace(FLOW_CLI + ".tmp", FLOW_CLI)
+    # the multiscale pyramid programs (the reference's decompose, recompose, merge_coarse)
+    for tool, name in enumerate(("decompose", "recompose", "merge_coarse")):
+        exe = os.path.join(os.path.dirname(CLI), name)
+        _run([CXX, *CXXFLAGS, f"-DPYR_TOOL={tool}", "-o", exe + ".tmp", os.path.join(HOST, "pyramid_main.cpp"),
+              f"-L{HERE}", "-l:libvbm3d_host.so", "-l:libvbm3d_hip.so", "-Wl,-rpath,$ORIGIN/.."], verbose)
+        os.replace(exe + ".tmp", exe)
     return LIB
 
 

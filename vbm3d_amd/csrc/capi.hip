@@ -22,6 +22,8 @@
 #include <vector>
 
 #include "../../include/vbm3d_hip.h"
+#include <rocblas/rocblas.h>
+
 #include "kernels.h"
 #include "select.h"
 
@@ -358,6 +360,8 @@ struct vbm3d_ctx {
     DevBuf idx, nsx, nsim, dist, est, wgt, tabs, refxy, meta, gpos, rlist, scal, stats, counts;
     DevBuf tabsg, mpos;  // general path: tables, members' slice corners
     DevBuf tv;           // TV-L1 flow workspace (flow.hip)
+    DevBuf pyr;          // pyramid workspace (pyramid.hip)
+    rocblas_handle blas = nullptr;  // the pyramid's DCT products
     size_t est_budget = 0;  // bytes for the per-patch rings; 0 = from free device memory
     unsigned chunk_frames = 0;  // per-clip path: frames per chunk; 0 = from the budget
     // kernel profiling (vbm3d_profile / vbm3d_profile_read)
@@ -1268,6 +1272,21 @@ hipError_t ctx_flow_workspace(vbm3d_ctx* c, size_t bytes, void** p) {
     return e;
 }
 int ctx_fail(int code, const std::string& msg) { return fail(code, msg); }
+hipError_t ctx_pyr_workspace(vbm3d_ctx* c, size_t bytes, void** p) {
+    const hipError_t e = ensure(c->pyr, bytes);
+    *p = c->pyr.p;
+    return e;
+}
+rocblas_handle ctx_blas(vbm3d_ctx* c) {
+    if (!c->blas) {
+        if (rocblas_create_handle(&c->blas) != rocblas_status_success) {
+            c->blas = nullptr;
+            return nullptr;
+        }
+    }
+    if (rocblas_set_stream(c->blas, c->stream) != rocblas_status_success) return nullptr;
+    return c->blas;
+}
 }  // namespace vb
 
 extern "C" {
@@ -1297,13 +1316,14 @@ void vbm3d_destroy(vbm3d_ctx* c) {
                       &c->idx, &c->nsx, &c->nsim, &c->dist, &c->est, &c->wgt, &c->tabs,
                       &c->refxy, &c->meta, &c->gpos, &c->rlist, &c->scal, &c->stats, &c->counts,
                       &c->b_idx, &c->b_nsx, &c->b_est, &c->b_wgt, &c->b_refxy, &c->b_meta, &c->b_gpos,
-                      &c->b_tabs, &c->b_plan, &c->b_cnt, &c->b_save, &c->tabsg, &c->mpos, &c->tv})
+                      &c->b_tabs, &c->b_plan, &c->b_cnt, &c->b_save, &c->tabsg, &c->mpos, &c->tv, &c->pyr})
         if (b->p) (void)hipFree(b->p);
     if (c->b_exec) (void)hipGraphExecDestroy(c->b_exec);
     if (c->b_graph) (void)hipGraphDestroy(c->b_graph);
     for (hipEvent_t e : c->ready)
         if (e) (void)hipEventDestroy(e);
     if (c->side) (void)hipStreamDestroy(c->side);
+    if (c->blas) (void)rocblas_destroy_handle(c->blas);
     (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -1632,6 +1652,68 @@ int vbm3d_set_chunk_frames(vbm3d_ctx* ctx, unsigned frames) {
     if (!ctx) return fail(VBM3D_ERR_ARG, "invalid argument");
     ctx->chunk_frames = frames;
     return VBM3D_OK;
+}
+
+int vbm3d_bench_transforms(int K, int mode, long npatch, int reps, double* ms, double* max_err) {
+    if ((K != 7 && K != 8) || (mode != 0 && mode != 1) || npatch <= 0 || npatch > (1L << 26) || reps <= 0 || !ms ||
+        !max_err)
+        return fail(VBM3D_ERR_ARG, "k must be 7 or 8, mode 0 (VALU) or 1 (MFMA), 0 < npatch <= 2^26, reps > 0");
+    Tabs tb;
+    fill_tabs(tb, (unsigned)K);
+    XTabs t{};
+    std::memcpy(t.F, tb.dctF, sizeof t.F);
+    std::memcpy(t.I, tb.dctI, sizeof t.I);
+    std::memcpy(t.nrm, tb.nrm, sizeof t.nrm);
+    std::memcpy(t.inv, tb.inv, sizeof t.inv);
+    t.dct_out = tb.dct_out;
+    for (int u = 0; u < K; u++)
+        for (int q = 0; q < K; q++)
+            t.M[u][q] = (float)((u == 0 ? std::sqrt(1.0 / K) : std::sqrt(2.0 / K)) *
+                                std::cos(M_PI * u * (2.0 * q + 1.0) / (2.0 * K)));
+    const size_t n = (size_t)npatch * K * K;
+    std::vector<float> h(n), o(n);
+    uint32_t st = 12345u;
+    for (size_t i = 0; i < n; i++) {
+        st = st * 1664525u + 1013904223u;
+        h[i] = (float)(st >> 8) * (255.0f / 16777216.0f);
+    }
+    DevBuf din, dout, dt;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    auto done = [&](int rc) {
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        for (DevBuf* b : {&din, &dout, &dt})
+            if (b->p) (void)hipFree(b->p);
+        return rc;
+    };
+    if (ensure(din, n * 4) != hipSuccess || ensure(dout, n * 4) != hipSuccess || ensure(dt, sizeof(XTabs)) != hipSuccess)
+        return done(fail(VBM3D_ERR_NOMEM, "device allocation failed"));
+    const float* in = (const float*)din.p;
+    float* out = (float*)dout.p;
+    const XTabs* tp = (const XTabs*)dt.p;
+    if (hipMemcpy(din.p, h.data(), n * 4, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(dt.p, &t, sizeof t, hipMemcpyHostToDevice) != hipSuccess ||
+        launch_xform_bench(K, mode, in, out, npatch, 1, tp, nullptr) != hipSuccess ||
+        hipDeviceSynchronize() != hipSuccess ||
+        hipMemcpy(o.data(), dout.p, n * 4, hipMemcpyDeviceToHost) != hipSuccess)
+        return done(fail(VBM3D_ERR_HIP, "transform benchmark: launch or copy failed"));
+    double err = 0;
+    for (size_t i = 0; i < n; i++) err = std::max(err, (double)std::fabs(o[i] - h[i]));
+    *max_err = err;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+        return done(fail(VBM3D_ERR_HIP, "hipEventCreate failed"));
+    std::vector<double> times;
+    for (int k = 0; k < 4; k++) {
+        float f = 0;
+        if (hipEventRecord(e0, nullptr) != hipSuccess || launch_xform_bench(K, mode, in, out, npatch, reps, tp, nullptr) != hipSuccess ||
+            hipEventRecord(e1, nullptr) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+            hipEventElapsedTime(&f, e0, e1) != hipSuccess)
+            return done(fail(VBM3D_ERR_HIP, "transform benchmark: timed launch failed"));
+        if (k > 0) times.push_back(f);
+    }
+    std::sort(times.begin(), times.end());
+    *ms = times[1];
+    return done(VBM3D_OK);
 }
 
 int vbm3d_profile(vbm3d_ctx* ctx, int enable) {

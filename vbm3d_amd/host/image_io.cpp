@@ -517,6 +517,19 @@ bool load_frame(const std::string& path, std::vector<float>& planar, unsigned& w
     return true;
 }
 
+bool load_planar(const std::string& path, std::vector<float>& planar, unsigned& w, unsigned& h, unsigned& c,
+                 std::string& err)
+{
+    std::vector<float> il;
+    bool bytes;
+    if (!load_interleaved(path, il, w, h, c, bytes, err)) return false;
+    planar.resize(il.size());
+    for (unsigned ch = 0, i = 0; ch < c; ch++)
+        for (unsigned y = 0; y < h; y++)
+            for (unsigned x = 0; x < w; x++, i++) planar[i] = il[((size_t)y * w + x) * c + ch];
+    return true;
+}
+
 bool load_grey(const std::string& path, std::vector<float>& img, unsigned& w, unsigned& h, std::string& err)
 {
     std::vector<float> il;

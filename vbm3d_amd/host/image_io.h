@@ -20,6 +20,10 @@ bool load_frame(const std::string& path, std::vector<float>& planar, unsigned& w
                 unsigned& c, std::string& err);
 bool save_frame(const std::string& path, const std::vector<float>& planar, unsigned w, unsigned h,
                 unsigned c, std::string& err);
+// Every channel as the file holds it, planar [c][y][x] (the pyramid
+// programs' reader, multiscaler/multiscaler.cpp:121-127: no channel rule).
+bool load_planar(const std::string& path, std::vector<float>& planar, unsigned& w, unsigned& h, unsigned& c,
+                 std::string& err);
 // One grey plane [y][x] as the flow program reads its images
 // (tvl1flow/iio.c:2683-2702, 808-871): RGB and RGBA become
 // .299 R + .587 G + .114 B in double, truncated to an integer for 8-bit
