@@ -159,6 +159,9 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="batched: default launch configuration")
     ap.add_argument("--no-batched", action="store_true",
                     help="skip the batched-form ratio in the default line (config.batched)")
+    ap.add_argument("--no-flow", action="store_true", help="skip the TV-L1 flow line (config.flow)")
+    ap.add_argument("--no-transforms", action="store_true",
+                    help="skip the VALU-against-MFMA transform benchmark (config.transforms)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -203,7 +206,89 @@ def main():
         res["config"]["batched"] = {k: b["config"][k] for k in ("workload", "per_frame_ms", "per_clip_ms",
                                                                 "batched_ms", "over_per_clip", "bit_identical")}
         res["config"]["batched"]["over_per_frame"] = b["value"]
+    if not args.no_transforms:
+        print("[bench] DCT transforms, VALU against MFMA", file=sys.stderr, flush=True)
+        res["config"]["transforms"] = measure_transforms(api)
+    if not args.no_flow:
+        print("[bench] TV-L1 flow", file=sys.stderr, flush=True)
+        res["config"]["flow"] = measure_flow(api, synth, torch, args, cpu=not args.no_cpu)
     print(json.dumps(res), flush=True)
+
+
+def measure_transforms(api, npatch=1 << 20, reps=64):
+    """The group filter's k x k DCT (k = 7 for step 2, 8 for step 1) on
+    the vector ALUs, as the denoiser computes it, against FP32 MFMA
+    (v_mfma_f32_4x4x1f32, k padded to 8): 2D transforms per second with
+    the patches on chip, and each form's largest round-trip error."""
+    out = {}
+    for K in (7, 8):
+        row = {}
+        for mode, name in ((0, "valu"), (1, "mfma")):
+            ms, rate, err = api.bench_transforms(K, mode, npatch, reps)
+            row[name] = {"ms": round(ms, 3), "transforms_per_s": float(f"{rate:.4g}"),
+                         "round_trip_max_err": float(f"{err:.3g}")}
+        row["mfma_over_valu"] = round(row["mfma"]["transforms_per_s"] / row["valu"]["transforms_per_s"], 3)
+        out[f"k{K}"] = row
+    out["workload"] = f"{npatch} random patches, {reps} forward + inverse 2D DCT round trips per launch"
+    return out
+
+
+def measure_flow(api, synth, torch, args, cpu=True):
+    """The TV-L1 flow (the reference's tvl1flow with its command-line
+    defaults) from frame 0 to frame 1 of the headline clip, as grey
+    images (.299 R + .587 G + .114 B), inputs resident in HBM: seconds per
+    flow (median of 3 after a warm-up), the iterations of every level and
+    warp, and the algorithmic HBM rate of the iterations (16 floats read
+    and 6 written per pixel and iteration).  cpu: the oracle (sequential,
+    bit-identical to the reference's library) on a 640x360 crop of the
+    same pair."""
+    W, H = args.width, args.height
+    _, noisy = synth.noisy_clip(W, H, 2, args.channels, args.sigma, args.seed, vmax=args.vmax)
+    if noisy.shape[1] == 3:
+        grey = (0.299 * noisy[:, 0].astype(np.float64) + 0.587 * noisy[:, 1] + 0.114 * noisy[:, 2]).astype(np.float32)
+    else:
+        grey = noisy[:, 0].copy()
+    p = api.tvl1_params()
+    p.nscales = api.tvl1_nscales(W, H, p.zfactor, p.nscales)
+    d0 = torch.from_numpy(grey[0]).to("cuda:0")
+    d1 = torch.from_numpy(grey[1]).to("cuda:0")
+    du, dv = torch.empty_like(d0), torch.empty_like(d0)
+    torch.cuda.synchronize()
+    ctx = api.Context(0)
+    iters = api.tvl1_flow_device(d0, d1, du, dv, p, ctx=ctx)
+    times = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        iters = api.tvl1_flow_device(d0, d1, du, dv, p, ctx=ctx)  # returns after the flow is done
+        times.append(time.perf_counter() - t0)
+    ctx.close()
+    dt = sorted(times)[1]
+    # level sizes as the pyramid makes them (tvl1flow/zoom.c:24-36)
+    sizes = [(W, H)]
+    for _ in range(1, p.nscales):
+        w, h = sizes[-1]
+        sizes.append((int(float(np.float32(w) * np.float32(p.zfactor)) + 0.5),
+                      int(float(np.float32(h) * np.float32(p.zfactor)) + 0.5)))
+    solved = list(range(p.nscales - 1, p.fscale - 1, -1))
+    it_bytes = sum(88.0 * sizes[lv][0] * sizes[lv][1] * int(iters[k].sum()) for k, lv in enumerate(solved))
+    out = {"workload": f"TV-L1 flow, {W}x{H} grey pair (frames 0 and 1 of the headline clip), "
+                       f"tau 0.25 lambda 0.15 theta 0.3, {p.nscales} levels, zfactor 0.5, 5 warps, epsilon 0.01",
+           "ms": round(dt * 1e3, 3), "mpix_s": round(W * H / dt / 1e6, 3),
+           "iterations_per_level": [int(x) for x in iters.sum(axis=1)],
+           "iteration_bytes": it_bytes, "iteration_gbps": round(it_bytes / dt / 1e9, 1)}
+    if cpu:
+        from oracle import tvl1 as T
+        cw, ch = min(640, W), min(360, H)
+        x0, y0 = (W - cw) // 2, (H - ch) // 2
+        a, b = grey[0, y0:y0 + ch, x0:x0 + cw], grey[1, y0:y0 + ch, x0:x0 + cw]
+        t0 = time.perf_counter()
+        T.flow(a, b)
+        tc = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": round(cw * ch / tc / 1e6, 4), "unit": "Mpixel/s", "cores": 1,
+                               "kind": "port", "ms": round(tc * 1e3, 1),
+                               "sample": f"{cw}x{ch} centre crop of the same pair, the oracle (sequential C++, "
+                                         "bit-identical to the reference's tvl1flow_lib.c)"}
+    return out
 
 
 def measure_batched(api, synth, torch, args, tune=None):

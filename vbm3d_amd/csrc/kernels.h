@@ -283,6 +283,18 @@ inline int bm_shape(const BMArgs& a) { return a.Ns == 7 && a.Npr == 5 && a.Nb ==
 hipError_t launch_bm(int K, const BMArgs& a, hipStream_t s);
 hipError_t launch_filter(int K, int step, const FArgs& a, hipStream_t s);
 hipError_t launch_aggregate(int C, const AArgs& a, unsigned frames, hipStream_t s);
+
+// Transform micro-benchmark (xform_bench.hip): the k x k DCT on the vector
+// ALUs with the denoiser's tables (mode 0) or on the matrix cores with the
+// orthonormal matrix M (mode 1); `reps` forward + inverse round trips.
+struct XTabs {
+    float F[64], I[64];      // REDFT10 / REDFT01 tables (Tabs::dctF, dctI)
+    float nrm[64], inv[64];  // coef_norm and coef_norm_inv
+    float dct_out;           // 1 / (2k)
+    float M[8][8];           // orthonormal DCT-II matrix, zero-padded to 8x8
+};
+hipError_t launch_xform_bench(int K, int mode, const float* in, float* out, long npatch, int reps, const XTabs* t,
+                              hipStream_t s);
 hipError_t launch_color(float* v, unsigned W, unsigned H, unsigned F, bool forward,
                         hipStream_t s);
 hipError_t launch_filter_adjoint(int K, const FArgs& a, hipStream_t s);

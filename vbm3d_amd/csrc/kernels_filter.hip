@@ -119,83 +119,6 @@ __device__ __forceinline__ float wiener_sum(const float* vc) {
     return acc;
 }
 
-// One 1D pass of REDFT10 on K values: row v of the matrix is even (v
-// even) or odd (v odd) about the centre, so each output takes the folded
-// inputs x[q] + x[K-1-q] or x[q] - x[K-1-q], q < K/2, plus the centre
-// sample for odd K and even v; sums in q order, as the oracle's
-// dct_pass_fwd.  About half the multiplications of the matrix product.
-template <int K, class TAB>
-__device__ __forceinline__ void dct_fwd_pass(const float* x, float* y, const TAB* F) {
-    constexpr int H = K / 2;
-    float s[H], d[H];
-#pragma unroll
-    for (int q = 0; q < H; q++) {
-        s[q] = x[q] + x[K - 1 - q];
-        d[q] = x[q] - x[K - 1 - q];
-    }
-#pragma unroll
-    for (int v = 0; v < K; v++) {
-        float m = 0.f;
-        if ((v & 1) == 0) {
-#pragma unroll
-            for (int q = 0; q < H; q++) m = m + s[q] * F[v * K + q];
-            if (K & 1) m = m + x[H] * F[v * K + H];
-        } else {
-#pragma unroll
-            for (int q = 0; q < H; q++) m = m + d[q] * F[v * K + q];
-        }
-        y[v] = m;
-    }
-}
-
-// Output v of dct_fwd_pass on x[0], x[stride], ... (the same folding and
-// the same order of operations, so the two agree bit for bit).
-template <int K, class TAB>
-__device__ __forceinline__ float dct_fwd_one(const float* x, int stride, int v, const TAB* F) {
-    constexpr int H = K / 2;
-    float m = 0.f;
-    if ((v & 1) == 0) {
-#pragma unroll
-        for (int q = 0; q < H; q++) m = m + (x[q * stride] + x[(K - 1 - q) * stride]) * F[v * K + q];
-        if (K & 1) m = m + x[H * stride] * F[v * K + H];
-    } else {
-#pragma unroll
-        for (int q = 0; q < H; q++) m = m + (x[q * stride] - x[(K - 1 - q) * stride]) * F[v * K + q];
-    }
-    return m;
-}
-
-// One 1D pass of REDFT01 (I[o][i] = table[o*K + i]): rows o and K-1-o
-// differ only in the sign of the odd-i terms, so E (even i) and O (odd i)
-// give y[o] = E + O and y[K-1-o] = E - O; the centre row of odd K is E.
-// Sums in i order, as the oracle's dct_pass_inv.
-template <int K, class TAB>
-__device__ __forceinline__ void dct_inv_pass(const float* z, float* y, const TAB* I) {
-    constexpr int H = K / 2;
-#pragma unroll
-    for (int o = 0; o < H; o++) {
-        float e = 0.f, od = 0.f;
-#pragma unroll
-        for (int i = 0; i < K; i += 2) e = e + z[i] * I[o * K + i];
-#pragma unroll
-        for (int i = 1; i < K; i += 2) od = od + z[i] * I[o * K + i];
-        y[o] = e + od;
-        y[K - 1 - o] = e - od;
-    }
-    if (K & 1) {
-        float e = 0.f;
-#pragma unroll
-        for (int i = 0; i < K; i += 2) e = e + z[i] * I[H * K + i];
-        y[H] = e;
-    }
-}
-
-// Transposition buffer: patch n of a lane batch is a K x (K+1) tile at
-// n*xstride(K) (row pitch K+1).  The strides keep both the row-wise write and
-// the column-wise read of the 64 lanes (n, r) = n*K + r bank-conflict free
-// at K = 8 and within 2-way at K = 7.
-__host__ __device__ constexpr int xstride(int K) { return (K & 1) ? K * (K + 1) + 1 : K * (K + 1); }
-
 // Per-wavefront LDS layout of k_filter (4-byte words): transposes, zg,
 // eg, then one v^2 buffer per channel, each region 16-byte aligned.
 __host__ __device__ constexpr int round4(int v) { return (v + 3) & ~3; }
