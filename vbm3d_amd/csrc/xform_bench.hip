@@ -126,13 +126,17 @@ __global__ __launch_bounds__(256) void k_xform_mfma(const float* __restrict__ in
     const int p = lane >> 3, r = lane & 7, j = lane & 3;
     const bool act = r < K && base + p < npatch;
     // forward: out = M X M^T, B[q][c] = M[c][q]; inverse: B[q][c] = M[q][c]
+    // flat indices into the 8x8 table (row-major): written as t->M[a][b]
+    // these loads compiled wrongly with this toolchain (rows 5-7 of the
+    // forward table came from row 0 of the inverse's loads)
+    const float* Mf = &t->M[0][0];
     float tf[2][8], ti[2][8];
 #pragma unroll
     for (int cb = 0; cb < 2; cb++)
 #pragma unroll
         for (int q = 0; q < 8; q++) {
-            tf[cb][q] = t->M[4 * cb + j][q];
-            ti[cb][q] = t->M[q][4 * cb + j];
+            tf[cb][q] = Mf[(4 * cb + j) * 8 + q];
+            ti[cb][q] = Mf[q * 8 + 4 * cb + j];
         }
     float v[8];
 #pragma unroll
