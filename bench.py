@@ -162,6 +162,8 @@ def main():
     ap.add_argument("--no-flow", action="store_true", help="skip the TV-L1 flow line (config.flow)")
     ap.add_argument("--no-transforms", action="store_true",
                     help="skip the VALU-against-MFMA transform benchmark (config.transforms)")
+    ap.add_argument("--flow-only", action="store_true",
+                    help="instead: only the TV-L1 flow measurement (the config.flow object)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -174,6 +176,9 @@ def main():
 
     if args.batched:
         print(json.dumps(measure_batched(api, synth, torch, args)), flush=True)
+        return
+    if args.flow_only:
+        print(json.dumps(measure_flow(api, synth, torch, args, cpu=not args.no_cpu)), flush=True)
         return
 
     W, H, F, C, sigma = args.width, args.height, args.frames, args.channels, args.sigma
