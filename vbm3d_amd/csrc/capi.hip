@@ -492,7 +492,8 @@ int run_step(vbm3d_ctx* ctx, const StepIO& io, hipStream_t s) {
     unsigned ring = std::min<unsigned>(Fr, S + 2 * p.Nf + (kt - 1));
     if (S >= Fr) { S = Fr; ring = Fr; }
     const size_t slots = (size_t)ring * nrf;
-    if ((slots * kMaxN) >> 32) return fail(VBM3D_ERR_UNSUPPORTED, "too many reference patches in flight");
+    // slot * kMaxN + n and the `est` plane index slot * C * N + n are 32-bit
+    if ((slots * kMaxN * 3) >> 32) return fail(VBM3D_ERR_UNSUPPORTED, "too many reference patches in flight");
 
     HIP_TRY(ensure(ctx->idx, slots * N * 4));
     HIP_TRY(ensure(ctx->nsx, slots * 4));
@@ -929,7 +930,7 @@ int plan_batch(vbm3d_ctx* ctx, const vbm3d_clip* clips, int n, int nb, const vbm
                 eb += slots * m.C * m.N * m.K * m.K;
                 wb += slots * m.C;
             }
-            if (((size_t)sb * kMaxN) >> 32) return fail(VBM3D_ERR_UNSUPPORTED, "too many reference patches in flight");
+            if (((size_t)sb * kMaxN * 3) >> 32) return fail(VBM3D_ERR_UNSUPPORTED, "too many reference patches in flight");
             need_slots = std::max(need_slots, sb);
             need_idx = std::max(need_idx, ib);
             need_est = std::max(need_est, eb);

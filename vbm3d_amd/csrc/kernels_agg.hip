@@ -47,10 +47,11 @@ __device__ __forceinline__ void agg_tile(const A& a, const int bx, const int by,
     constexpr int QS = 8;             // a wavefront's block edge
     constexpr int NWV = NT / 64, WX = TX / QS;
     static_assert(TX % QS == 0 && TY % QS == 0 && NT % 64 == 0, "tile of 8x8 blocks");
-    // the ordered list of one NT-patch chunk: at most nl >= N members each
-    extern __shared__ uint32_t agg_lists[];
-    uint32_t* L_sn = agg_lists;
-    uint32_t* L_pos = agg_lists + NT * nl;
+    // the ordered list of one NT-patch chunk (at most nl >= N members
+    // each): entry = {position x | y << 16, slot * kMaxN + n}, one 8-byte
+    // LDS read per entry
+    extern __shared__ uint2 agg_lists[];
+    uint2* L_e = agg_lists;
     __shared__ float s_kai[kMaxWin * kMaxWin * 4];
     __shared__ int s_w[NWV];
     const int tid = threadIdx.x;
@@ -150,8 +151,7 @@ __device__ __forceinline__ void agg_tile(const A& a, const int bx, const int by,
                 if ((hits >> n) & 1u) {
                     const unsigned r = mi[n] - tbase;
                     const unsigned qy = div32(r, dW), qx = r - qy * W;
-                    L_sn[o] = slot * kMaxN + n;
-                    L_pos[o] = qx | (qy << 16);
+                    L_e[o] = make_uint2(qx | (qy << 16), slot * kMaxN + n);
                     o++;
                 }
             __syncthreads();
@@ -160,7 +160,7 @@ __device__ __forceinline__ void agg_tile(const A& a, const int bx, const int by,
             for (int j0 = 0; j0 < tot; j0 += 64) {
                 bool ov = false;
                 if (j0 + lane < tot) {
-                    const uint32_t pos = L_pos[j0 + lane];
+                    const uint32_t pos = L_e[j0 + lane].x;
                     const int ex = (int)(pos & 0xFFFFu), ey = (int)(pos >> 16);
                     ov = ex <= qx0 + QS - 1 && ex + K - 1 >= qx0 && ey <= qy0 + QS - 1 && ey + K - 1 >= qy0;
                 }
@@ -186,8 +186,9 @@ __device__ __forceinline__ void agg_tile(const A& a, const int bx, const int by,
 #pragma unroll
                     for (int u = 0; u < U; u++) {
                         const int jv = jj[u] >= 0 ? jj[u] : jj[0];
-                        const uint32_t pos = __builtin_amdgcn_readfirstlane(L_pos[jv]);
-                        const uint32_t sn = __builtin_amdgcn_readfirstlane(L_sn[jv]);
+                        const uint2 ent = L_e[jv];
+                        const uint32_t pos = __builtin_amdgcn_readfirstlane(ent.x);
+                        const uint32_t sn = __builtin_amdgcn_readfirstlane(ent.y);
                         const uint32_t sl = sn / kMaxN, n = sn % kMaxN;
                         const cfloat* wb = (const cfloat*)(a.wgt + (size_t)sl * C);
 #pragma unroll
@@ -196,13 +197,21 @@ __device__ __forceinline__ void agg_tile(const A& a, const int bx, const int by,
                         on[u] = (jj[u] >= 0) & inside & ((unsigned)dx < (unsigned)K) & ((unsigned)dy < (unsigned)K);
                         const unsigned kk = on[u] ? (unsigned)(dy * K + dx) : 0u;
                         kai[u] = s_kai[kk];
-                        // wave-uniform base, 32-bit lane byte offset
-                        const char* eb = (const char*)(a.est + ((size_t)sl * C * N + n) * K2);
+                        // wave-uniform base, 32-bit lane byte offset; the
+                        // plane index slot * C * N + n fits 32 bits (the
+                        // host refuses larger rings)
+                        const unsigned pl = sl * (unsigned)(C * N) + n;
+                        const char* eb = (const char*)a.est + (size_t)pl * (unsigned)(K2 * 4);
 #pragma unroll
                         for (int c = 0; c < C; c++)
                             ev[u][c] = *(const float*)(eb + (size_t)c * N * K2 * 4 + kk * 4u);
                     }
                     // keeps the loads above from being sunk to their sums
+                    // (the weights' scalar loads included)
+#pragma unroll
+                    for (int u = 0; u < U; u++)
+#pragma unroll
+                        for (int c = 0; c < C; c++) asm volatile("" ::"s"(w[u][c]));
                     asm volatile("" ::: "memory");
 #pragma unroll
                     for (int u = 0; u < U; u++) {
@@ -365,7 +374,7 @@ __global__ void k_flow_absmax(const float* fl, size_t n, unsigned* out) {
 hipError_t launch_aggregate_multi(int C, const AArgs* jobs, const WorkQueue& q, int maxN, hipStream_t s) {
     if (q.total <= 0) return hipSuccess;
     if (maxN < 1 || maxN > kMaxN || q.total > 0x7FFFFFFFL) return hipErrorInvalidValue;
-    const size_t lds = (size_t)2 * kAggThreads * maxN * sizeof(uint32_t);
+    const size_t lds = (size_t)kAggThreads * maxN * sizeof(uint2);
     const dim3 grid((unsigned)q.total), blk(kAggThreads);
     if (maxN <= 8) {
         if (C == 1) hipLaunchKernelGGL((k_aggregate_multi<1, 8>), grid, blk, lds, s, jobs, q, maxN);
@@ -380,7 +389,7 @@ hipError_t launch_aggregate_multi(int C, const AArgs* jobs, const WorkQueue& q, 
 hipError_t launch_aggregate_queue(int C, const AArgs* jobs, const WorkQueue& q, int blocks, int maxN, hipStream_t s) {
     if (q.total <= 0 || blocks <= 0) return hipSuccess;
     if (maxN < 1 || maxN > kMaxN) return hipErrorInvalidValue;
-    const size_t lds = (size_t)2 * kAggThreads * maxN * sizeof(uint32_t);
+    const size_t lds = (size_t)kAggThreads * maxN * sizeof(uint2);
     const dim3 blk(kAggThreads);
     // members held in registers by the scan: 8, or 16 when some job has N = 16
     if (maxN <= 8) {
@@ -403,7 +412,7 @@ hipError_t launch_aggregate(int C, const AArgs& a, unsigned frames, hipStream_t 
     if (frames == 0) return hipSuccess;
     dim3 grid((a.W + kAggTileX - 1) / kAggTileX, (a.H + kAggTileY - 1) / kAggTileY, frames);
     if (a.N < 1 || a.N > kMaxN) return hipErrorInvalidValue;
-    const size_t lds = (size_t)2 * kAggThreads * a.N * sizeof(uint32_t);  // the chunk's ordered list
+    const size_t lds = (size_t)kAggThreads * a.N * sizeof(uint2);  // the chunk's ordered list
     const dim3 blk(kAggThreads);
     // members held in registers by the scan: 8, or 16 for N = 16
 #define VB_AGG(NM)                                                                         \
