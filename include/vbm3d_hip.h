@@ -316,7 +316,7 @@ int vbm3d_pyr_merge_coarse(vbm3d_ctx* ctx, const float* fine, int rows,
 
 /* The multiscale denoising of a video [F][C][H][W]: every frame into its
  * levels, every level denoised as a video by run_vbm3d with sigma / 2^level
- * truncated to two decimals (the driver script's `bc`) and default
+ * truncated to two decimals (the multiscale shell script's `bc`) and default
  * parameters for that sigma (or p1 / p2 when not NULL), every frame
  * recomposed.  noisy_levels / denoised_levels (optional) receive the
  * levels, level after level, each [F][C][rows_i][cols_i]. */

@@ -109,7 +109,7 @@ def test_multiscale_denoise_pipeline(gpu_ctx, type_):
 
 def test_pyramid_command_lines(gpu_ctx, tmp_path):
     """bin/decompose, bin/recompose and bin/merge_coarse on float TIFF
-    files: the same levels and images as the API (the driver script's
+    files: the same levels and images as the API (the multiscale script's
     invocations, src/utils/VBM3Ddenoising_multiscale.sh:45-75)."""
     import os
     import subprocess

@@ -1,6 +1,6 @@
-// capi.hip -- host driver and C ABI of libvbm3d_hip.so.
+// capi.hip -- host orchestration and C ABI of libvbm3d_hip.so.
 //
-// The driver replaces run_vbm3d's thread tiling and per-row loops
+// The orchestration replaces run_vbm3d's thread tiling and per-row loops
 // (src/vbm3d.cpp:87-376, 527-653, 759-927) with a frame-chunked pipeline
 // on one HIP stream:
 //     for each chunk of reference frames:

@@ -1,6 +1,6 @@
 // kernels.h -- device-side argument blocks and launch helpers shared by
 // the HIP kernels (kernels_bm.hip, kernels_filter.hip, kernels_agg.hip)
-// and the host driver (capi.hip).
+// and the host orchestration (capi.hip).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>

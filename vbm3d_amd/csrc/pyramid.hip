@@ -614,7 +614,7 @@ int vbm3d_multiscale_denoise(vbm3d_ctx* ctx, float sigma, const float* noisy, un
     }
     PYR_TRY(hipStreamSynchronize(P.s));
     // 2. each level denoised as a video with sigma / 2^level, truncated to
-    //    two decimals as the driver script's `bc` computes it
+    //    two decimals as the multiscale shell script's `bc` computes it
     //    (src/utils/VBM3Ddenoising_multiscale.sh:52-58)
     for (int i = 0; i < levels; i++) {
         const double sl_exact = (double)sigma / std::ldexp(1.0, i);
