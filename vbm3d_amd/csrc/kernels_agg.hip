@@ -180,7 +180,9 @@ __device__ __forceinline__ void agg_tile(const A& a, const int bx, const int by,
                     }
                     // straight-line loads: an absent entry repeats the
                     // first one, a lane outside a patch reads the patch's
-                    // first word; only the sums below are masked
+                    // sample nearest to it (within the rows and columns the
+                    // patch's own lanes read, so no extra cache line); only
+                    // the sums below are masked
                     float ev[U][C], kai[U], w[U][C];
                     bool on[U];
 #pragma unroll
@@ -195,7 +197,7 @@ __device__ __forceinline__ void agg_tile(const A& a, const int bx, const int by,
                         for (int c = 0; c < C; c++) w[u][c] = RAW ? 0.f : wb[c];
                         const int dx = px - (int)(pos & 0xFFFFu), dy = py - (int)(pos >> 16);
                         on[u] = (jj[u] >= 0) & inside & ((unsigned)dx < (unsigned)K) & ((unsigned)dy < (unsigned)K);
-                        const unsigned kk = on[u] ? (unsigned)(dy * K + dx) : 0u;
+                        const unsigned kk = (unsigned)(min(max(dy, 0), K - 1) * K + min(max(dx, 0), K - 1));
                         kai[u] = s_kai[kk];
                         // wave-uniform base, 32-bit lane byte offset; the
                         // plane index slot * C * N + n fits 32 bits (the
