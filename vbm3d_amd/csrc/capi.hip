@@ -340,6 +340,21 @@ int member_reach(const vbm3d_params& p, unsigned maxflow, unsigned W, unsigned H
     return (int)std::min<unsigned long long>(r, std::max(W, H));
 }
 
+// The same reach per frame distance: a member k frames from its reference
+// patch lies within R0 + k Rs of it (the reference-frame window, then k
+// predictive windows and flow steps), never beyond member_reach.  Options
+// whose trajectories drift (mc with kt > 1) keep the single bound.
+void member_reach_steps(const vbm3d_params& p, unsigned maxflow, unsigned W, unsigned H, int* R0, int* Rs) {
+    const int R = member_reach(p, maxflow, W, H);
+    if (p.mc && p.kt > 1) {
+        *R0 = R;
+        *Rs = 0;
+        return;
+    }
+    *R0 = std::min<int>((int)(p.Ns - 1) / 2, R);
+    *Rs = (int)std::min<unsigned long long>((p.Npr - 1) / 2 + (unsigned long long)maxflow, (unsigned long long)R);
+}
+
 }  // namespace
 
 struct ProfEvent {
@@ -560,6 +575,7 @@ int run_step(vbm3d_ctx* ctx, const StepIO& io, hipStream_t s) {
     ag.idx = bm.idx_out; ag.nsx = bm.nsx_out;
     ag.est = fa.est; ag.wgt = fa.wgt; ag.tabs = fa.tabs;
     ag.R = member_reach(p, io.maxflow, io.W, io.H);
+    member_reach_steps(p, io.maxflow, io.W, io.H, &ag.R0, &ag.Rs);
     ag.divW = make_div32(io.W);
     ag.fallback = io.fallback;
     ag.out = io.out;
@@ -1052,6 +1068,7 @@ int plan_batch(vbm3d_ctx* ctx, const vbm3d_clip* clips, int n, int nb, const vbm
                         ag.wgt = d_wgt + m.wgt_base;
                         ag.tabs = d_tabs + m.tab;
                         ag.R = member_reach(p, m.maxflow, c.W, c.H);
+                        member_reach_steps(p, m.maxflow, c.W, c.H, &ag.R0, &ag.Rs);
                         ag.divW = make_div32(c.W);
                         ag.t_begin = m.t_done;
                         ag.fallback = st == 1 ? c.noisy : (nb == 1 ? c.basic : c.noisy);

@@ -165,6 +165,7 @@ struct AArgs {
     const float* wgt;
     const Tabs* tabs;
     int R;                      // largest member-to-reference displacement
+    int R0, Rs;                 // a member k frames away lies within min(R, R0 + k Rs)
     Div32 divW;                 // division by W
     unsigned t_begin;           // first output frame of this launch
     const float* fallback;      // used where den == 0

@@ -68,24 +68,54 @@ __device__ __forceinline__ void agg_tile(const A& a, const int bx, const int by,
 #pragma unroll
     for (int c = 0; c < C; c++) tn[c] = td[c] = 0.f;
     const int trlo = max(0, (int)to - a.Nf), trhi = min((int)a.Fr - 1, (int)to + a.Nf);
-    const int rlo = y0 - a.R - (K - 1), rhi = y0 + TY - 1 + a.R;
-    const int clo = x0 - a.R - (K - 1), chi = x0 + TX - 1 + a.R;
+    const int nfr = trhi - trlo + 1;
     // a member at q = to*whc + y*W + x overlaps the tile's rows iff
     // q - to*whc lies in [ylo*W, yhi*W)
     const unsigned tbase = to * whc;
     const Div32 dW = {a.divW.m, a.divW.s1, a.divW.s2};
     const unsigned rmin = (unsigned)max(0, y0 - K + 1) * W, rmax = (unsigned)min((int)a.H, y0 + TY) * W;
 
+    // per frame of a crop's box: {first index, columns, first row, first
+    // column} of the grid patches whose members can reach the tile, and the
+    // division by the column count
+    __shared__ int4 s_box[2 * kGenMaxNf + 1];
+    __shared__ Div32 s_div[2 * kGenMaxNf + 1];
+    __shared__ int s_total;
+
     for (int ci = 0; ci < a.ncrop; ci++) {
         const int* meta = a.crop_meta + ci * 5;
         const int off = meta[0], nrows = meta[1], ncols = meta[2];
         const uint16_t* rows = a.grid_pos + meta[3];
         const uint16_t* cols = a.grid_pos + meta[4];
-        const int ia = lower_bound_u16(rows, nrows, rlo), ib = lower_bound_u16(rows, nrows, rhi + 1);
-        const int ja = lower_bound_u16(cols, ncols, clo), jb = lower_bound_u16(cols, ncols, chi + 1);
-        const int nr = ib - ia, nc = jb - ja, ntr = trhi - trlo + 1;
-        const long total = (nr > 0 && nc > 0 && ntr > 0) ? (long)ntr * nr * nc : 0;
-        const Div32 dnc = make_div32(nc > 0 ? nc : 1), dnr = make_div32(nr > 0 ? nr : 1);
+        // A member k frames away from its reference patch lies within
+        // min(R, R0 + k Rs) of it, so the box narrows towards the output
+        // frame.  The frames' boxes are concatenated in frame order: the
+        // scan below still visits the patches in processing order (crop,
+        // frame, row, column).
+        __syncthreads();  // the previous crop's table is no longer read
+        if (tid < nfr) {
+            const int k = abs(trlo + tid - (int)to);
+            const int Rk = (int)min((long)a.R, (long)a.R0 + (long)k * a.Rs);
+            const int ia = lower_bound_u16(rows, nrows, y0 - Rk - (K - 1));
+            const int ib = lower_bound_u16(rows, nrows, y0 + TY + Rk);
+            const int ja = lower_bound_u16(cols, ncols, x0 - Rk - (K - 1));
+            const int jb = lower_bound_u16(cols, ncols, x0 + TX + Rk);
+            const int nr = max(ib - ia, 0), nc = max(jb - ja, 0);
+            s_box[tid] = make_int4(nr * nc, nc, ia, ja);
+            s_div[tid] = make_div32(nc > 0 ? nc : 1);
+        }
+        __syncthreads();
+        if (tid == 0) {
+            int acc = 0;
+            for (int q = 0; q < nfr; q++) {
+                const int cnt = s_box[q].x;
+                s_box[q].x = acc;
+                acc += cnt;
+            }
+            s_total = acc;
+        }
+        __syncthreads();
+        const long total = s_total;
         float pn[C], pd[C];
 #pragma unroll
         for (int c = 0; c < C; c++) pn[c] = pd[c] = 0.f;
@@ -102,10 +132,18 @@ __device__ __forceinline__ void agg_tile(const A& a, const int bx, const int by,
 #pragma unroll
             for (int n = 0; n < NM; n++) mi_n[n] = 0u;
             if (e < total) {
-                const unsigned rest = div32(e, dnc), jj = e - rest * nc;
-                const unsigned tq = div32(rest, dnr), ii = rest - tq * nr;
-                const unsigned tr = trlo + tq;
-                slot_n = (tr % a.ring) * a.nrf + off + (ia + ii) * ncols + (ja + jj);
+                // the frame whose box holds index e: the last one starting
+                // at or before it (empty boxes start where the next does)
+                int q = 0;
+#pragma unroll
+                for (int st = 16; st >= 1; st >>= 1)
+                    if (q + st < nfr && (int)e >= s_box[q + st].x) q += st;
+                const int4 bx = s_box[q];
+                const Div32 dq = s_div[q];
+                const unsigned rest = e - (unsigned)bx.x;
+                const unsigned ii = div32(rest, dq), jj = rest - ii * (unsigned)bx.y;
+                const unsigned tr = trlo + q;
+                slot_n = (tr % a.ring) * a.nrf + off + (bx.z + ii) * ncols + (bx.w + jj);
                 ns_n = (int)a.nsx[slot_n];
                 const uint32_t* mem = a.idx + (size_t)slot_n * N;
 #pragma unroll
