@@ -265,8 +265,11 @@ __device__ __forceinline__ void filter_one(const A& a, const long r, float* X, c
         if (lane == c) wsel = wc;
         wave_sync();
         // ---- inverse 2D transforms (bior_2d_inverse src/lib_transforms.cpp:
-        //      270-321; dct_2d_inv src/vbm3d.cpp:1678-1713), store [n][pixel]
-        const float* est = STEP == 2 ? eg : zg;
+        //      270-321; dct_2d_inv src/vbm3d.cpp:1678-1713), store [n][pixel].
+        //      A member's outputs replace its coefficients in LDS (pass 1
+        //      of its batch has read them all), and the group's planes then
+        //      leave in contiguous 256-byte stores.
+        float* est = STEP == 2 ? eg : zg;
         float* dstg = a.est + (((size_t)slot * C + c) * a.N) * K2;
         for (int nb0 = 0; nb0 < nsx; nb0 += PB) {
             const int n = nb0 + nl;
@@ -304,17 +307,19 @@ __device__ __forceinline__ void filter_one(const A& a, const long r, float* X, c
                         float m = 0.f;
 #pragma unroll
                         for (int p = 0; p < K; p++) m = m + col[p] * TI[u * K + p];
-                        dstg[(size_t)n * K2 + u * K + rr] = m;
+                        est[n * K2 + u * K + rr] = m;
                     }
                 } else {
                     float y[K];
                     dct_inv_pass<K>(col, y, TI);
 #pragma unroll
-                    for (int u = 0; u < K; u++) dstg[(size_t)n * K2 + u * K + rr] = dct_out * y[u];
+                    for (int u = 0; u < K; u++) est[n * K2 + u * K + rr] = dct_out * y[u];
                 }
             }
             wave_sync();
         }
+        for (int j = lane; j < nsx * K2; j += 64) dstg[j] = est[j];
+        wave_sync();  // read before the next channel's transforms refill it
     }
     if (STEP == 2) {
         // Wiener weights: sequential float sums over the flat index
