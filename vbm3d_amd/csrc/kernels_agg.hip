@@ -40,7 +40,9 @@ __device__ __forceinline__ int lower_bound_u16(const uint16_t* a, int n, int v) 
 // list, and each wavefront then accumulates, for its 8x8 block, the
 // members that overlap the block in list order: every output sample sees
 // its contributions in exactly the reference's order.
-template <int C, bool RAW, int NM, class A>
+// ONE: the grid is a single crop (nb = 1); the per-crop sums are then the
+// totals, and no crop-order merge is kept live across the scan.
+template <int C, bool RAW, int NM, bool ONE, class A>
 __device__ __forceinline__ void agg_tile(const A& a, const int bx, const int by, const unsigned bz, const int nl) {
     typedef const __attribute__((address_space(4))) float cfloat;
     constexpr int TX = kAggTileX, TY = kAggTileY, NT = kAggThreads;
@@ -82,7 +84,7 @@ __device__ __forceinline__ void agg_tile(const A& a, const int bx, const int by,
     __shared__ Div32 s_div[2 * kGenMaxNf + 1];
     __shared__ int s_total;
 
-    for (int ci = 0; ci < a.ncrop; ci++) {
+    for (int ci = 0; ci < (ONE ? 1 : a.ncrop); ci++) {
         const int* meta = a.crop_meta + ci * 5;
         const int off = meta[0], nrows = meta[1], ncols = meta[2];
         const uint16_t* rows = a.grid_pos + meta[3];
@@ -273,7 +275,7 @@ __device__ __forceinline__ void agg_tile(const A& a, const int bx, const int by,
         }
 #pragma unroll
         for (int c = 0; c < C; c++) {
-            if (a.ncrop == 1) {
+            if (ONE) {
                 tn[c] = pn[c];
                 td[c] = pd[c];
             } else {
@@ -298,7 +300,8 @@ __device__ __forceinline__ void agg_tile(const A& a, const int bx, const int by,
 // than at its unconstrained 7 waves.
 template <int C, bool RAW, int NM>
 __global__ __launch_bounds__(kAggThreads) __attribute__((amdgpu_waves_per_eu(8))) void k_aggregate(AArgs a) {
-    agg_tile<C, RAW, NM>(a, blockIdx.x, blockIdx.y, blockIdx.z, a.N);
+    if (a.ncrop == 1) agg_tile<C, RAW, NM, true>(a, blockIdx.x, blockIdx.y, blockIdx.z, a.N);
+    else agg_tile<C, RAW, NM, false>(a, blockIdx.x, blockIdx.y, blockIdx.z, a.N);
 }
 
 // Batched form with one workgroup per tile: workgroup b takes item b, so
@@ -315,7 +318,8 @@ __global__ __launch_bounds__(kAggThreads) __attribute__((amdgpu_waves_per_eu(8))
     const long local = it - uniform_long(q.item_begin[j]);
     const long tx = (a.W + kAggTileX - 1) / kAggTileX, ty = (a.H + kAggTileY - 1) / kAggTileY;
     const long bz = local / (tx * ty), rem = local - bz * tx * ty;
-    agg_tile<C, false, NM>(a, (int)(rem % tx), (int)(rem / tx), (unsigned)bz, nl);
+    if (a.ncrop == 1) agg_tile<C, false, NM, true>(a, (int)(rem % tx), (int)(rem / tx), (unsigned)bz, nl);
+    else agg_tile<C, false, NM, false>(a, (int)(rem % tx), (int)(rem / tx), (unsigned)bz, nl);
 }
 
 // Persistent form: one tile per take (the workgroup is the unit); item ->
@@ -335,7 +339,8 @@ __global__ __launch_bounds__(kAggThreads) void k_aggregate_queue(const AArgs* __
         const long local = it - q.item_begin[j];
         const long tx = (a.W + kAggTileX - 1) / kAggTileX, ty = (a.H + kAggTileY - 1) / kAggTileY;
         const long bz = local / (tx * ty), rem = local - bz * tx * ty;
-        agg_tile<C, false, NM>(a, (int)(rem % tx), (int)(rem / tx), (unsigned)bz, nl);
+        if (a.ncrop == 1) agg_tile<C, false, NM, true>(a, (int)(rem % tx), (int)(rem / tx), (unsigned)bz, nl);
+        else agg_tile<C, false, NM, false>(a, (int)(rem % tx), (int)(rem / tx), (unsigned)bz, nl);
     }
 }
 
