@@ -513,12 +513,18 @@ void dct_pass_fwd(const T* x, unsigned sx, T* y, unsigned sy, const TabT<T>& tb)
         d[q] = x[q * sx] - x[(k - 1 - q) * sx];
     }
     for (unsigned v = 0; v < k; v++) {
-        T m = 0;
+        T m;
         if (v % 2 == 0) {
-            for (unsigned q = 0; q < h; q++) m += s[q] * tb.dctF[v * k + q];
-            if (k & 1) m += x[h * sx] * tb.dctF[v * k + h];
+            if (h == 0) {
+                m = x[0] * tb.dctF[v * k];  // k = 1: the centre sample alone
+            } else {
+                m = s[0] * tb.dctF[v * k];
+                for (unsigned q = 1; q < h; q++) m += s[q] * tb.dctF[v * k + q];
+                if (k & 1) m += x[h * sx] * tb.dctF[v * k + h];
+            }
         } else {
-            for (unsigned q = 0; q < h; q++) m += d[q] * tb.dctF[v * k + q];
+            m = d[0] * tb.dctF[v * k];
+            for (unsigned q = 1; q < h; q++) m += d[q] * tb.dctF[v * k + q];
         }
         y[v * sy] = m;
     }
@@ -532,15 +538,15 @@ template <class T>
 void dct_pass_inv(const T* z, unsigned sz, T* y, unsigned sy, const TabT<T>& tb) {
     const unsigned k = tb.k, h = k / 2;
     for (unsigned o = 0; o < h; o++) {
-        T e = 0, od = 0;
-        for (unsigned i = 0; i < k; i += 2) e += z[i * sz] * tb.dctI[o * k + i];
-        for (unsigned i = 1; i < k; i += 2) od += z[i * sz] * tb.dctI[o * k + i];
+        T e = z[0] * tb.dctI[o * k], od = z[sz] * tb.dctI[o * k + 1];
+        for (unsigned i = 2; i < k; i += 2) e += z[i * sz] * tb.dctI[o * k + i];
+        for (unsigned i = 3; i < k; i += 2) od += z[i * sz] * tb.dctI[o * k + i];
         y[o * sy] = e + od;
         y[(k - 1 - o) * sy] = e - od;
     }
     if (k & 1) {
-        T e = 0;
-        for (unsigned i = 0; i < k; i += 2) e += z[i * sz] * tb.dctI[h * k + i];
+        T e = z[0] * tb.dctI[h * k];
+        for (unsigned i = 2; i < k; i += 2) e += z[i * sz] * tb.dctI[h * k + i];
         y[h * sy] = e;
     }
 }

@@ -276,14 +276,16 @@ __device__ __forceinline__ void t3_inv(float* v, bool had) {
     else haar_inv<NS>(v);
 }
 
-// One 1D pass of REDFT10 on K values: row v of the matrix is even (v
-// even) or odd (v odd) about the centre, so each output takes the folded
-// inputs x[q] + x[K-1-q] or x[q] - x[K-1-q], q < K/2, plus the centre
-// sample for odd K and even v; sums in q order, as the oracle's
-// dct_pass_fwd.  About half the multiplications of the matrix product.
+// One 1D pass of REDFT10 on K values (K >= 2): row v of the matrix is
+// even (v even) or odd (v odd) about the centre, so each output takes the
+// folded inputs x[q] + x[K-1-q] or x[q] - x[K-1-q], q < K/2, plus the
+// centre sample for odd K and even v; sums in q order starting from the
+// first product, as the oracle's dct_pass_fwd.  About half the
+// multiplications of the matrix product.
 template <int K, class TAB>
 __device__ __forceinline__ void dct_fwd_pass(const float* x, float* y, const TAB* F) {
     constexpr int H = K / 2;
+    static_assert(H >= 1, "a pass of at least two samples");
     float s[H], d[H];
 #pragma unroll
     for (int q = 0; q < H; q++) {
@@ -292,14 +294,16 @@ __device__ __forceinline__ void dct_fwd_pass(const float* x, float* y, const TAB
     }
 #pragma unroll
     for (int v = 0; v < K; v++) {
-        float m = 0.f;
+        float m;
         if ((v & 1) == 0) {
+            m = s[0] * F[v * K];
 #pragma unroll
-            for (int q = 0; q < H; q++) m = m + s[q] * F[v * K + q];
+            for (int q = 1; q < H; q++) m = m + s[q] * F[v * K + q];
             if (K & 1) m = m + x[H] * F[v * K + H];
         } else {
+            m = d[0] * F[v * K];
 #pragma unroll
-            for (int q = 0; q < H; q++) m = m + d[q] * F[v * K + q];
+            for (int q = 1; q < H; q++) m = m + d[q] * F[v * K + q];
         }
         y[v] = m;
     }
@@ -310,39 +314,44 @@ __device__ __forceinline__ void dct_fwd_pass(const float* x, float* y, const TAB
 template <int K, class TAB>
 __device__ __forceinline__ float dct_fwd_one(const float* x, int stride, int v, const TAB* F) {
     constexpr int H = K / 2;
-    float m = 0.f;
+    static_assert(H >= 1, "a pass of at least two samples");
+    float m;
     if ((v & 1) == 0) {
+        m = (x[0] + x[(K - 1) * stride]) * F[v * K];
 #pragma unroll
-        for (int q = 0; q < H; q++) m = m + (x[q * stride] + x[(K - 1 - q) * stride]) * F[v * K + q];
+        for (int q = 1; q < H; q++) m = m + (x[q * stride] + x[(K - 1 - q) * stride]) * F[v * K + q];
         if (K & 1) m = m + x[H * stride] * F[v * K + H];
     } else {
+        m = (x[0] - x[(K - 1) * stride]) * F[v * K];
 #pragma unroll
-        for (int q = 0; q < H; q++) m = m + (x[q * stride] - x[(K - 1 - q) * stride]) * F[v * K + q];
+        for (int q = 1; q < H; q++) m = m + (x[q * stride] - x[(K - 1 - q) * stride]) * F[v * K + q];
     }
     return m;
 }
 
-// One 1D pass of REDFT01 (I[o][i] = table[o*K + i]): rows o and K-1-o
-// differ only in the sign of the odd-i terms, so E (even i) and O (odd i)
-// give y[o] = E + O and y[K-1-o] = E - O; the centre row of odd K is E.
-// Sums in i order, as the oracle's dct_pass_inv.
+// One 1D pass of REDFT01 (I[o][i] = table[o*K + i], K >= 2): rows o and
+// K-1-o differ only in the sign of the odd-i terms, so E (even i) and O
+// (odd i) give y[o] = E + O and y[K-1-o] = E - O; the centre row of odd K
+// is E.  Sums in i order starting from the first product, as the oracle's
+// dct_pass_inv.
 template <int K, class TAB>
 __device__ __forceinline__ void dct_inv_pass(const float* z, float* y, const TAB* I) {
     constexpr int H = K / 2;
+    static_assert(H >= 1, "a pass of at least two samples");
 #pragma unroll
     for (int o = 0; o < H; o++) {
-        float e = 0.f, od = 0.f;
+        float e = z[0] * I[o * K], od = z[1] * I[o * K + 1];
 #pragma unroll
-        for (int i = 0; i < K; i += 2) e = e + z[i] * I[o * K + i];
+        for (int i = 2; i < K; i += 2) e = e + z[i] * I[o * K + i];
 #pragma unroll
-        for (int i = 1; i < K; i += 2) od = od + z[i] * I[o * K + i];
+        for (int i = 3; i < K; i += 2) od = od + z[i] * I[o * K + i];
         y[o] = e + od;
         y[K - 1 - o] = e - od;
     }
     if (K & 1) {
-        float e = 0.f;
+        float e = z[0] * I[H * K];
 #pragma unroll
-        for (int i = 0; i < K; i += 2) e = e + z[i] * I[H * K + i];
+        for (int i = 2; i < K; i += 2) e = e + z[i] * I[H * K + i];
         y[H] = e;
     }
 }

@@ -284,12 +284,16 @@ __device__ void gen_fwd2d(const float* P, float* T, float* out, int K, bool bior
     for (int e = lane; e < K2; e += 64) {
         const int p = e / K, v = e - p * K;
         const float* x = P + p * K;
-        float m = 0.f;
-        if ((v & 1) == 0) {
-            for (int q = 0; q < h; q++) m = m + (x[q] + x[K - 1 - q]) * tb.dctF[v * K + q];
+        float m;
+        if (h == 0) {
+            m = x[0] * tb.dctF[v * K];  // K = 1: the centre sample alone
+        } else if ((v & 1) == 0) {
+            m = (x[0] + x[K - 1]) * tb.dctF[v * K];
+            for (int q = 1; q < h; q++) m = m + (x[q] + x[K - 1 - q]) * tb.dctF[v * K + q];
             if (K & 1) m = m + x[h] * tb.dctF[v * K + h];
         } else {
-            for (int q = 0; q < h; q++) m = m + (x[q] - x[K - 1 - q]) * tb.dctF[v * K + q];
+            m = (x[0] - x[K - 1]) * tb.dctF[v * K];
+            for (int q = 1; q < h; q++) m = m + (x[q] - x[K - 1 - q]) * tb.dctF[v * K + q];
         }
         T[e] = m;
     }
@@ -297,12 +301,16 @@ __device__ void gen_fwd2d(const float* P, float* T, float* out, int K, bool bior
     // columns: out[u][v] = REDFT10 of column v, times coef_norm
     for (int e = lane; e < K2; e += 64) {
         const int u = e / K, v = e - u * K;
-        float m = 0.f;
-        if ((u & 1) == 0) {
-            for (int q = 0; q < h; q++) m = m + (T[q * K + v] + T[(K - 1 - q) * K + v]) * tb.dctF[u * K + q];
+        float m;
+        if (h == 0) {
+            m = T[v] * tb.dctF[u * K];  // K = 1
+        } else if ((u & 1) == 0) {
+            m = (T[v] + T[(K - 1) * K + v]) * tb.dctF[u * K];
+            for (int q = 1; q < h; q++) m = m + (T[q * K + v] + T[(K - 1 - q) * K + v]) * tb.dctF[u * K + q];
             if (K & 1) m = m + T[h * K + v] * tb.dctF[u * K + h];
         } else {
-            for (int q = 0; q < h; q++) m = m + (T[q * K + v] - T[(K - 1 - q) * K + v]) * tb.dctF[u * K + q];
+            m = (T[v] - T[(K - 1) * K + v]) * tb.dctF[u * K];
+            for (int q = 1; q < h; q++) m = m + (T[q * K + v] - T[(K - 1 - q) * K + v]) * tb.dctF[u * K + q];
         }
         out[e] = m * tb.nrm[e];
     }
@@ -314,11 +322,11 @@ __device__ void gen_fwd2d(const float* P, float* T, float* out, int K, bool bior
 __device__ __forceinline__ float gen_redft01(const float* z, int sz, int o, int K, const TabsG& tb) {
     const int h = K / 2;
     const int r = o < K - 1 - o ? o : K - 1 - o;
-    float e = 0.f;
-    for (int i = 0; i < K; i += 2) e = e + z[i * sz] * tb.dctI[r * K + i];
+    float e = z[0] * tb.dctI[r * K];
+    for (int i = 2; i < K; i += 2) e = e + z[i * sz] * tb.dctI[r * K + i];
     if ((K & 1) && o == h) return e;
-    float od = 0.f;
-    for (int i = 1; i < K; i += 2) od = od + z[i * sz] * tb.dctI[r * K + i];
+    float od = z[sz] * tb.dctI[r * K + 1];
+    for (int i = 3; i < K; i += 2) od = od + z[i * sz] * tb.dctI[r * K + i];
     return o == r ? e + od : e - od;
 }
 
