@@ -144,6 +144,16 @@ __device__ __forceinline__ void bm_one(const A& a, const long r, float* base, co
             const int aw = wx + K - 1, ah = wy + K - 1;
             const unsigned fb = rt * whc + (unsigned)y0 * W + (unsigned)x0;
             const unsigned tl0 = lds_addr(tile);
+            // one scalar base per channel, kept opaque so that the copies
+            // take the scalar-base form (no 64-bit vector address per copy)
+            const char* cb[C];
+#pragma unroll
+            for (int c = 0; c < C; c++) {
+                unsigned long long b = (unsigned long long)(const char*)(vid + fb + c * wh);
+                unsigned lo = (unsigned)b, hi = (unsigned)(b >> 32);
+                asm volatile("" : "+s"(lo), "+s"(hi));
+                cb[c] = (const char*)(((unsigned long long)hi << 32) | lo);
+            }
 #pragma unroll
             for (int k = 0; k < 4; k++) {
                 if (k * 64 >= cs) break;
@@ -152,8 +162,7 @@ __device__ __forceinline__ void bm_one(const A& a, const long r, float* base, co
                 if (er < ah && ec < aw) {
                     const unsigned go = ((unsigned)er * W + (unsigned)ec) * 4u;
 #pragma unroll
-                    for (int c = 0; c < C; c++)
-                        lds_dma((const char*)(vid + fb + c * wh) + go, tl0 + (unsigned)((c * cs + k * 64) * 4));
+                    for (int c = 0; c < C; c++) lds_dma(cb[c] + go, tl0 + (unsigned)((c * cs + k * 64) * 4));
                 }
             }
             __builtin_amdgcn_s_waitcnt(0);
@@ -267,14 +276,28 @@ __device__ __forceinline__ void bm_one(const A& a, const long r, float* base, co
                         const unsigned fb = (unsigned)__builtin_amdgcn_readfirstlane(win[wn * 5 + 4]) * whc +
                                             (unsigned)__builtin_amdgcn_readfirstlane(win[wn * 5 + 1]) * W +
                                             (unsigned)__builtin_amdgcn_readfirstlane(win[wn * 5]);
+                        // one scalar base per channel, kept opaque so that
+                        // the copies take the scalar-base form (no 64-bit
+                        // vector address arithmetic per copy)
+                        const char* cb[C];
+#pragma unroll
+                        for (int c = 0; c < C; c++) {
+                            unsigned long long b = (unsigned long long)(const char*)(vid + fb + c * wh);
+                            unsigned lo = (unsigned)b, hi = (unsigned)(b >> 32);
+                            asm volatile("" : "+s"(lo), "+s"(hi));
+                            cb[c] = (const char*)(((unsigned long long)hi << 32) | lo);
+                        }
 #pragma unroll
                         for (int k = 0; k < 3; k++) {
                             if (k * 64 >= ne) break;
                             if (er[k] < ah && ec[k] < aw) {
+                                // a 32-bit lane offset at the copy (not a
+                                // hoisted 64-bit one) keeps the scalar-base form
+                                unsigned go = goff[k];
+                                asm volatile("" : "+v"(go));
 #pragma unroll
                                 for (int c = 0; c < C; c++)
-                                    lds_dma((const char*)(vid + fb + c * wh) + goff[k],
-                                            tl + (unsigned)((wn * L.ws + c * cs1 + k * 64) * 4));
+                                    lds_dma(cb[c] + go, tl + (unsigned)((wn * L.ws + c * cs1 + k * 64) * 4));
                             }
                         }
                     }
