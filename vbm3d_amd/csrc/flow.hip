@@ -257,62 +257,45 @@ __device__ __forceinline__ float div_at(const float* __restrict__ v1, const floa
     return -v1[p - 1] + v2[p] - v2[p - nx];
 }
 
-// ---- iteration, part 1 (tvl1flow_lib.c:170-231): thresholding towards
-// the data term, divergence of the dual field, primal update in place;
-// each workgroup leaves its sum of squared changes (double, fixed order).
-__global__ void k_iter_primal(const FlowCtl* __restrict__ ctl, float* u1, float* u2, const float* __restrict__ wx,
-                              const float* __restrict__ wy, const float* __restrict__ grad,
-                              const float* __restrict__ rho_c, const float* __restrict__ p11,
-                              const float* __restrict__ p12, const float* __restrict__ p21,
-                              const float* __restrict__ p22, double* part, int nx, int ny, float l_t, float theta) {
-    if (!ctl->active) return;  // uniform
-    const int j = blockIdx.x * BX + threadIdx.x, i = blockIdx.y * BY + threadIdx.y;
-    double e = 0.0;
-    if (j < nx && i < ny) {
-        const size_t p = (size_t)i * nx + j;
-        const float a = u1[p], b = u2[p], gx = wx[p], gy = wy[p], g = grad[p];
-        const float rho = rho_c[p] + (gx * a + gy * b);
-        float d1, d2;
-        if (rho < -l_t * g) {
-            d1 = l_t * gx;
-            d2 = l_t * gy;
-        } else if (rho > l_t * g) {
-            d1 = -l_t * gx;
-            d2 = -l_t * gy;
-        } else if ((double)g < 1E-10) {
-            d1 = d2 = 0.f;
-        } else {
-            const float fi = -rho / g;
-            d1 = fi * gx;
-            d2 = fi * gy;
-        }
-        const float v1 = a + d1, v2 = b + d2;
-        const float n1 = v1 + theta * div_at(p11, p12, i, j, nx, ny);
-        const float n2 = v2 + theta * div_at(p21, p22, i, j, nx, ny);
-        u1[p] = n1;
-        u2[p] = n2;
-        e = (double)((n1 - a) * (n1 - a) + (n2 - b) * (n2 - b));
+// ---- iteration, part 1 (tvl1flow_lib.c:170-231) at pixel (i, j):
+// thresholding towards the data term, divergence of the dual field, primal
+// update in place; returns the squared change of the flow.
+__device__ __forceinline__ double primal_px(float* u1, float* u2, const float* __restrict__ wx,
+                                            const float* __restrict__ wy, const float* __restrict__ grad,
+                                            const float* __restrict__ rho_c, const float* __restrict__ p11,
+                                            const float* __restrict__ p12, const float* __restrict__ p21,
+                                            const float* __restrict__ p22, int i, int j, int nx, int ny, float l_t,
+                                            float theta) {
+    const size_t p = (size_t)i * nx + j;
+    const float a = u1[p], b = u2[p], gx = wx[p], gy = wy[p], g = grad[p];
+    const float rho = rho_c[p] + (gx * a + gy * b);
+    float d1, d2;
+    if (rho < -l_t * g) {
+        d1 = l_t * gx;
+        d2 = l_t * gy;
+    } else if (rho > l_t * g) {
+        d1 = -l_t * gx;
+        d2 = -l_t * gy;
+    } else if ((double)g < 1E-10) {
+        d1 = d2 = 0.f;
+    } else {
+        const float fi = -rho / g;
+        d1 = fi * gx;
+        d2 = fi * gy;
     }
-    for (int o = 32; o >= 1; o >>= 1) e += __shfl_xor(e, o, 64);
-    __shared__ double s[BX * BY / 64];
-    const int t = threadIdx.y * BX + threadIdx.x;
-    if ((t & 63) == 0) s[t >> 6] = e;
-    __syncthreads();
-    if (t == 0) {
-        double acc = 0.0;
-        for (int q = 0; q < BX * BY / 64; q++) acc += s[q];
-        part[blockIdx.y * gridDim.x + blockIdx.x] = acc;
-    }
+    const float v1 = a + d1, v2 = b + d2;
+    const float n1 = v1 + theta * div_at(p11, p12, i, j, nx, ny);
+    const float n2 = v2 + theta * div_at(p21, p22, i, j, nx, ny);
+    u1[p] = n1;
+    u2[p] = n2;
+    return (double)((n1 - a) * (n1 - a) + (n2 - b) * (n2 - b));
 }
 
-// ---- iteration, part 2 (tvl1flow_lib.c:233-252): forward gradient of
-// the new flow and the projected dual update.
-__global__ void k_iter_dual(const FlowCtl* __restrict__ ctl, const float* __restrict__ u1,
-                            const float* __restrict__ u2, float* p11, float* p12, float* p21, float* p22, int nx,
-                            int ny, float taut) {
-    if (!ctl->active) return;  // uniform
-    const int j = blockIdx.x * BX + threadIdx.x, i = blockIdx.y * BY + threadIdx.y;
-    if (j >= nx || i >= ny) return;
+// ---- iteration, part 2 (tvl1flow_lib.c:233-252) at pixel (i, j):
+// forward gradient of the new flow and the projected dual update.
+__device__ __forceinline__ void dual_px(const float* __restrict__ u1, const float* __restrict__ u2, float* p11,
+                                        float* p12, float* p21, float* p22, int i, int j, int nx, int ny,
+                                        float taut) {
     const size_t p = (size_t)i * nx + j;
     const float u1x = j < nx - 1 ? u1[p + 1] - u1[p] : 0.f, u1y = i < ny - 1 ? u1[p + nx] - u1[p] : 0.f;
     const float u2x = j < nx - 1 ? u2[p + 1] - u2[p] : 0.f, u2y = i < ny - 1 ? u2[p + nx] - u2[p] : 0.f;
@@ -326,6 +309,40 @@ __global__ void k_iter_dual(const FlowCtl* __restrict__ ctl, const float* __rest
     p12[p] = (p12[p] + taut * u1y) / ng1;
     p21[p] = (p21[p] + taut * u2x) / ng2;
     p22[p] = (p22[p] + taut * u2y) / ng2;
+}
+
+// Part 1 over the level; each workgroup (64 x 4 pixels) leaves its sum of
+// squared changes in a fixed order: every row of 64 by a wavefront's
+// butterfly, then the four rows in order.
+__global__ void k_iter_primal(const FlowCtl* __restrict__ ctl, float* u1, float* u2, const float* __restrict__ wx,
+                              const float* __restrict__ wy, const float* __restrict__ grad,
+                              const float* __restrict__ rho_c, const float* __restrict__ p11,
+                              const float* __restrict__ p12, const float* __restrict__ p21,
+                              const float* __restrict__ p22, double* part, int nx, int ny, float l_t, float theta) {
+    if (!ctl->active) return;  // uniform
+    const int j = blockIdx.x * BX + threadIdx.x, i = blockIdx.y * BY + threadIdx.y;
+    double e = 0.0;
+    if (j < nx && i < ny) e = primal_px(u1, u2, wx, wy, grad, rho_c, p11, p12, p21, p22, i, j, nx, ny, l_t, theta);
+    for (int o = 32; o >= 1; o >>= 1) e += __shfl_xor(e, o, 64);
+    __shared__ double s[BX * BY / 64];
+    const int t = threadIdx.y * BX + threadIdx.x;
+    if ((t & 63) == 0) s[t >> 6] = e;
+    __syncthreads();
+    if (t == 0) {
+        double acc = 0.0;
+        for (int q = 0; q < BX * BY / 64; q++) acc += s[q];
+        part[blockIdx.y * gridDim.x + blockIdx.x] = acc;
+    }
+}
+
+// Part 2 over the level.
+__global__ void k_iter_dual(const FlowCtl* __restrict__ ctl, const float* __restrict__ u1,
+                            const float* __restrict__ u2, float* p11, float* p12, float* p21, float* p22, int nx,
+                            int ny, float taut) {
+    if (!ctl->active) return;  // uniform
+    const int j = blockIdx.x * BX + threadIdx.x, i = blockIdx.y * BY + threadIdx.y;
+    if (j >= nx || i >= ny) return;
+    dual_px(u1, u2, p11, p12, p21, p22, i, j, nx, ny, taut);
 }
 
 // ---- iteration, part 3: the mean squared change and the stopping test
@@ -354,6 +371,78 @@ __global__ void k_ctl_start(FlowCtl* ctl) {
     ctl->active = 1;
     ctl->iter = 0;
     ctl->error = INFINITY;
+}
+
+// Every iteration of one warp on a level of at most kSmallPx pixels (and
+// kSmallBlocks blocks of 64 x 4), in one workgroup of 1024 threads: the
+// same per-pixel updates as k_iter_primal and k_iter_dual, and the stopping
+// test's sums in their order (a row of 64 pixels by a wavefront's
+// butterfly, the four rows of a block in order, then the blocks' sums as
+// k_iter_ctl adds them), so the flow and the iteration count are those of
+// the three-launch form.  It replaces up to 3 x 300 launches and the host's
+// reads of the control word; the count goes to *count.
+constexpr int kSmallPx = 16384, kSmallBlocks = 256;
+__global__ __launch_bounds__(1024) void k_iter_small(FlowCtl* ctl, float* u1, float* u2, const float* __restrict__ wx,
+                                                     const float* __restrict__ wy, const float* __restrict__ grad,
+                                                     const float* __restrict__ rho_c, float* p11, float* p12,
+                                                     float* p21, float* p22, int nx, int ny, float l_t, float theta,
+                                                     float taut, float size, float eps2, int* count) {
+    __shared__ double s_row[kSmallBlocks * BY];
+    __shared__ double s_blk[kSmallBlocks];
+    __shared__ double s_w[4];
+    __shared__ int s_active, s_iter;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int gx = (nx + BX - 1) / BX, gy = (ny + BY - 1) / BY, nblk = gx * gy;
+    if (t == 0) {
+        s_active = ctl->active;
+        s_iter = ctl->iter;
+    }
+    for (;;) {
+        __syncthreads();  // the previous iteration's dual updates and control word
+        if (!s_active) break;  // workgroup-uniform
+        // primal, one row of 64 pixels (block b, row r) per wavefront step
+        for (int vr = w; vr < nblk * BY; vr += 16) {
+            const int b = vr / BY, r = vr - b * BY;
+            const int j = (b % gx) * BX + lane, i = (b / gx) * BY + r;
+            double e = 0.0;
+            if (j < nx && i < ny) e = primal_px(u1, u2, wx, wy, grad, rho_c, p11, p12, p21, p22, i, j, nx, ny, l_t, theta);
+            for (int o = 32; o >= 1; o >>= 1) e += __shfl_xor(e, o, 64);
+            if (lane == 0) s_row[vr] = e;
+        }
+        __syncthreads();
+        for (int b = t; b < nblk; b += 1024) {
+            double acc = 0.0;
+            for (int q = 0; q < BY; q++) acc += s_row[b * BY + q];
+            s_blk[b] = acc;
+        }
+        // dual, every pixel
+        for (int q = t; q < nx * ny; q += 1024) {
+            const int i = q / nx, j = q - i * nx;
+            dual_px(u1, u2, p11, p12, p21, p22, i, j, nx, ny, taut);
+        }
+        __syncthreads();
+        if (t < 256) {
+            double acc = 0.0;
+            for (int q = t; q < nblk; q += 256) acc += s_blk[q];
+            for (int o = 32; o >= 1; o >>= 1) acc += __shfl_xor(acc, o, 64);
+            if (lane == 0) s_w[w] = acc;
+        }
+        __syncthreads();
+        if (t == 0) {
+            double tt = 0.0;
+            for (int q = 0; q < 4; q++) tt += s_w[q];
+            float err = (float)tt;
+            err /= size;
+            s_iter += 1;
+            s_active = (err > eps2 && s_iter < kMaxIter) ? 1 : 0;
+            ctl->error = err;
+        }
+    }
+    if (t == 0) {
+        ctl->iter = s_iter;
+        ctl->active = 0;
+        *count = s_iter;
+    }
 }
 
 __global__ void k_fill(float* p, long n, float v) {
@@ -434,7 +523,8 @@ int tvl1_device(vbm3d_ctx* ctx, const float* I0, const float* I1, float* u, floa
     const size_t work_planes = 11;  // I1x I1y wx wy grad rho_c p11 p12 p21 p22 tmp
     // planes, then the iteration partials, the min/max partials and the
     // control word, each 256-byte aligned
-    const size_t bytes = (words + work_planes * n0) * 4 + (size_t)nparts * 8 + 256 * 4 + 4096;
+    const size_t bytes = (words + work_planes * n0) * 4 + (size_t)nparts * 8 + 256 * 4 + 4096 +
+                         (size_t)nscales * warps * 4;
     void* base = nullptr;
     if (ctx_flow_workspace(ctx, bytes, &base) != hipSuccess)
         return ctx_fail(VBM3D_ERR_NOMEM, "device allocation of the flow workspace failed");
@@ -461,6 +551,7 @@ int tvl1_device(vbm3d_ctx* ctx, const float* I0, const float* I1, float* u, floa
     double* part = (double*)(((uintptr_t)f + 255) & ~(uintptr_t)255);
     float* mm = (float*)(part + nparts);  // 64 workgroups x 4 min/max partials
     FlowCtl* ctl = (FlowCtl*)(((uintptr_t)(mm + 256) + 255) & ~(uintptr_t)255);
+    int* counts = (int*)(ctl + 1);  // iterations of the single-workgroup levels, [level][warp]
 
     // normalisation and 0.8-sigma pre-smoothing (tvl1flow_lib.c:416-421)
     hipLaunchKernelGGL(k_minmax, dim3(64), dim3(256), 0, s, I0, I1, (long)n0, mm);
@@ -499,11 +590,14 @@ int tvl1_device(vbm3d_ctx* ctx, const float* I0, const float* I1, float* u, floa
     };
     int level = 0;
     FlowCtl host_ctl{};
+    std::vector<char> small_level(nscales, 0);
     for (int k = nscales - 1; k >= fscale; k--, level++) {
         const int lx = L[k].nx, ly = L[k].ny;
         const long sz = (long)lx * ly;
         const dim3 g = grid2(lx, ly), b(BX, BY);
         const int np = (int)(g.x * g.y);
+        const bool small = sz <= kSmallPx && np <= kSmallBlocks;
+        small_level[level] = small;
         // one level (tvl1flow_lib.c:96-305)
         hipLaunchKernelGGL(k_cgrad, g, b, 0, s, L[k].I1, I1x, I1y, lx, ly);
         for (float* q : {p11, p12, p21, p22}) hipLaunchKernelGGL(k_fill, dim3(256), dim3(256), 0, s, q, sz, 0.f);
@@ -512,6 +606,13 @@ int tvl1_device(vbm3d_ctx* ctx, const float* I0, const float* I1, float* u, floa
             hipLaunchKernelGGL(k_warp, g, b, 0, s, L[k].I0, L[k].I1, I1x, I1y, L[k].u1, L[k].u2, wx, wy, grad, rho,
                                lx, ly);
             hipLaunchKernelGGL(k_ctl_start, dim3(1), dim3(1), 0, s, ctl);
+            if (small) {
+                hipLaunchKernelGGL(k_iter_small, dim3(1), dim3(1024), 0, s, ctl, L[k].u1, L[k].u2, wx, wy, grad, rho,
+                                   p11, p12, p21, p22, lx, ly, l_t, theta, taut, (float)sz, eps2,
+                                   counts + level * warps + w);
+                FLOW_TRY(hipGetLastError());
+                continue;
+            }
             int launched = 0;
             while (launched < kMaxIter) {
                 const int batch = std::min(kMaxIter - launched, launched == 0 ? 24 : 48);
@@ -535,6 +636,16 @@ int tvl1_device(vbm3d_ctx* ctx, const float* I0, const float* I1, float* u, floa
     }
     for (int k = fscale - 1; k >= 1; k--)
         if ((rc = upsample(k)) != VBM3D_OK) return rc;
+    if (iters) {
+        const int nlev = nscales - fscale;  // levels solved
+        std::vector<int> dev_counts((size_t)nlev * warps);
+        if (!dev_counts.empty())
+            FLOW_TRY(hipMemcpyAsync(dev_counts.data(), counts, dev_counts.size() * 4, hipMemcpyDeviceToHost, s));
+        FLOW_TRY(hipStreamSynchronize(s));
+        for (int lv = 0; lv < nlev; lv++)
+            if (small_level[lv])
+                for (int w = 0; w < warps; w++) iters[lv * warps + w] = dev_counts[(size_t)lv * warps + w];
+    }
     FLOW_TRY(hipStreamSynchronize(s));
     return VBM3D_OK;
 }
