@@ -81,6 +81,7 @@ __device__ __forceinline__ void agg_tile(const A& a, const int bx, const int by,
     // column} of the grid patches whose members can reach the tile, and the
     // division by the column count
     __shared__ int4 s_box[2 * kGenMaxNf + 1];
+    __shared__ int s_bnd[4 * (2 * kGenMaxNf + 1)];
     __shared__ Div32 s_div[2 * kGenMaxNf + 1];
     __shared__ int s_total;
 
@@ -95,13 +96,20 @@ __device__ __forceinline__ void agg_tile(const A& a, const int bx, const int by,
         // scan below still visits the patches in processing order (crop,
         // frame, row, column).
         __syncthreads();  // the previous crop's table is no longer read
-        if (tid < nfr) {
-            const int k = abs(trlo + tid - (int)to);
+        // the four searches of a frame run in four threads (each is a chain
+        // of dependent loads)
+        if (tid < 4 * nfr) {
+            const int q = tid >> 2, which = tid & 3;
+            const int k = abs(trlo + q - (int)to);
             const int Rk = (int)min((long)a.R, (long)a.R0 + (long)k * a.Rs);
-            const int ia = lower_bound_u16(rows, nrows, y0 - Rk - (K - 1));
-            const int ib = lower_bound_u16(rows, nrows, y0 + TY + Rk);
-            const int ja = lower_bound_u16(cols, ncols, x0 - Rk - (K - 1));
-            const int jb = lower_bound_u16(cols, ncols, x0 + TX + Rk);
+            const bool row = which < 2;
+            const int lo = row ? y0 - Rk - (K - 1) : x0 - Rk - (K - 1);
+            const int hi = row ? y0 + TY + Rk : x0 + TX + Rk;
+            s_bnd[tid] = lower_bound_u16(row ? rows : cols, row ? nrows : ncols, (which & 1) ? hi : lo);
+        }
+        __syncthreads();
+        if (tid < nfr) {
+            const int ia = s_bnd[4 * tid], ib = s_bnd[4 * tid + 1], ja = s_bnd[4 * tid + 2], jb = s_bnd[4 * tid + 3];
             const int nr = max(ib - ia, 0), nc = max(jb - ja, 0);
             s_box[tid] = make_int4(nr * nc, nc, ia, ja);
             s_div[tid] = make_div32(nc > 0 ? nc : 1);
