@@ -355,6 +355,38 @@ void member_reach_steps(const vbm3d_params& p, unsigned maxflow, unsigned W, uns
     *Rs = (int)std::min<unsigned long long>((p.Npr - 1) / 2 + (unsigned long long)maxflow, (unsigned long long)R);
 }
 
+// The aggregation's grid boxes (k_aggregate's layout of AArgs::boxtab):
+// per crop, for each tile row and frame distance k the first grid row at
+// or below y0 - R_k - (K - 1) and the first beyond y0 + TY - 1 + R_k, then
+// the same for the tile columns; R_k = min(R, R0 + k Rs).
+void box_table(const GridDef& g, unsigned W, unsigned H, int K, unsigned Nf, int R, int R0, int Rs,
+               std::vector<int2>& box) {
+    const int nty = (int)((H + kAggTileY - 1) / kAggTileY), ntx = (int)((W + kAggTileX - 1) / kAggTileX);
+    auto lb = [](const std::vector<unsigned>& v, long x) {
+        size_t lo = 0, hi = v.size();
+        while (lo < hi) {
+            const size_t mid = (lo + hi) / 2;
+            if ((long)v[mid] < x) lo = mid + 1;
+            else hi = mid;
+        }
+        return (int)lo;
+    };
+    box.clear();
+    box.reserve(g.rows.size() * (size_t)(nty + ntx) * (Nf + 1));
+    for (size_t c = 0; c < g.rows.size(); c++) {
+        for (int ty = 0; ty < nty; ty++)
+            for (unsigned k = 0; k <= Nf; k++) {
+                const long Rk = std::min<long>(R, (long)R0 + (long)k * Rs), y0 = (long)ty * kAggTileY;
+                box.push_back(make_int2(lb(g.rows[c], y0 - Rk - (K - 1)), lb(g.rows[c], y0 + kAggTileY + Rk)));
+            }
+        for (int tx = 0; tx < ntx; tx++)
+            for (unsigned k = 0; k <= Nf; k++) {
+                const long Rk = std::min<long>(R, (long)R0 + (long)k * Rs), x0 = (long)tx * kAggTileX;
+                box.push_back(make_int2(lb(g.cols[c], x0 - Rk - (K - 1)), lb(g.cols[c], x0 + kAggTileX + Rk)));
+            }
+    }
+}
+
 }  // namespace
 
 struct ProfEvent {
@@ -373,6 +405,7 @@ struct vbm3d_ctx {
     hipStream_t side = nullptr;
     hipEvent_t ready[3] = {nullptr, nullptr, nullptr};  // noisy final, basic final, upload chunk
     DevBuf idx, nsx, nsim, dist, est, wgt, tabs, refxy, meta, gpos, rlist, scal, stats, counts;
+    DevBuf boxtab;       // the aggregation's per-tile grid boxes
     DevBuf tabsg, mpos;  // general path: tables, members' slice corners
     DevBuf tv;           // TV-L1 flow workspace (flow.hip)
     DevBuf pyr;          // pyramid workspace (pyramid.hip)
@@ -485,7 +518,7 @@ int run_step(vbm3d_ctx* ctx, const StepIO& io, hipStream_t s) {
         // fixed ones end (the previous step's are no longer read)
         g_arena->used = g_arena->fixed;
         for (DevBuf* b : {&ctx->idx, &ctx->nsx, &ctx->est, &ctx->wgt, &ctx->tabs, &ctx->refxy, &ctx->meta, &ctx->gpos,
-                          &ctx->tabsg, &ctx->mpos})
+                          &ctx->tabsg, &ctx->mpos, &ctx->boxtab})
             *b = DevBuf{};
     }
     HIP_TRY(ensure(ctx->tabs, sizeof(Tabs)));
@@ -576,6 +609,17 @@ int run_step(vbm3d_ctx* ctx, const StepIO& io, hipStream_t s) {
     ag.est = fa.est; ag.wgt = fa.wgt; ag.tabs = fa.tabs;
     ag.R = member_reach(p, io.maxflow, io.W, io.H);
     member_reach_steps(p, io.maxflow, io.W, io.H, &ag.R0, &ag.Rs);
+    if (!gen) {
+        // each tile's grid box per frame distance, as the kernel would
+        // search it (one load per bound instead of a chain of dependent ones)
+        std::vector<int2> box;
+        box_table(g, io.W, io.H, (int)K, p.Nf, ag.R, ag.R0, ag.Rs, box);
+        HIP_TRY(ensure(ctx->boxtab, box.size() * sizeof(int2)));
+        HIP_TRY(hipMemcpyAsync(ctx->boxtab.p, box.data(), box.size() * sizeof(int2), hipMemcpyHostToDevice, s));
+        ag.boxtab = (const int2*)ctx->boxtab.p;
+        ag.nty = (int)((io.H + kAggTileY - 1) / kAggTileY);
+        ag.ntx = (int)((io.W + kAggTileX - 1) / kAggTileX);
+    }
     ag.divW = make_div32(io.W);
     ag.fallback = io.fallback;
     ag.out = io.out;
@@ -1268,8 +1312,11 @@ size_t step_scratch_bytes(const vbm3d_params& p, unsigned W, unsigned H, unsigne
     const bool gen = !fast_ok(p, W, H);
     const size_t Fr = F - p.kt + 1, ring = std::min<size_t>(Fr, 1 + 2 * (size_t)p.Nf + (p.kt - 1)), slots = ring * nrf;
     const size_t mpos_words = gen && p.kt > 1 ? (size_t)p.N * p.kt : 0;
+    const size_t nbox = gen ? 0
+                            : g.rows.size() * (size_t)((H + kAggTileY - 1) / kAggTileY + (W + kAggTileX - 1) / kAggTileX) *
+                                  (p.Nf + 1);
     return al(sizeof(Tabs)) + (gen ? al(sizeof(TabsG)) : 0) + al(nrf * 4) + al(g.rows.size() * 5 * 4) + al(npos * 2) +
-           slots * ((size_t)C * p.N * p.k * p.k * p.kt + p.N + 1 + C + mpos_words) * 4 + 4096;
+           al(nbox * sizeof(int2)) + slots * ((size_t)C * p.N * p.k * p.k * p.kt + p.N + 1 + C + mpos_words) * 4 + 4096;
 }
 
 }  // namespace
@@ -1334,7 +1381,8 @@ void vbm3d_destroy(vbm3d_ctx* c) {
                       &c->idx, &c->nsx, &c->nsim, &c->dist, &c->est, &c->wgt, &c->tabs,
                       &c->refxy, &c->meta, &c->gpos, &c->rlist, &c->scal, &c->stats, &c->counts,
                       &c->b_idx, &c->b_nsx, &c->b_est, &c->b_wgt, &c->b_refxy, &c->b_meta, &c->b_gpos,
-                      &c->b_tabs, &c->b_plan, &c->b_cnt, &c->b_save, &c->tabsg, &c->mpos, &c->tv, &c->pyr})
+                      &c->b_tabs, &c->b_plan, &c->b_cnt, &c->b_save, &c->tabsg, &c->mpos, &c->tv, &c->pyr,
+                      &c->boxtab})
         if (b->p) (void)hipFree(b->p);
     if (c->b_exec) (void)hipGraphExecDestroy(c->b_exec);
     if (c->b_graph) (void)hipGraphDestroy(c->b_graph);

@@ -96,9 +96,18 @@ __device__ __forceinline__ void agg_tile(const A& a, const int bx, const int by,
         // scan below still visits the patches in processing order (crop,
         // frame, row, column).
         __syncthreads();  // the previous crop's table is no longer read
-        // the four searches of a frame run in four threads (each is a chain
-        // of dependent loads)
-        if (tid < 4 * nfr) {
+        if (a.boxtab) {
+            // the host's table of the same bounds: one load per bound
+            if (tid < 4 * nfr) {
+                const int q = tid >> 2, which = tid & 3;
+                const int k = abs(trlo + q - (int)to);
+                const size_t tl = (size_t)ci * (a.nty + a.ntx) + (which < 2 ? by : a.nty + bx);
+                const int2 v = a.boxtab[tl * (a.Nf + 1) + k];
+                s_bnd[tid] = (which & 1) ? v.y : v.x;
+            }
+        } else if (tid < 4 * nfr) {
+            // the four searches of a frame run in four threads (each is a
+            // chain of dependent loads)
             const int q = tid >> 2, which = tid & 3;
             const int k = abs(trlo + q - (int)to);
             const int Rk = (int)min((long)a.R, (long)a.R0 + (long)k * a.Rs);
