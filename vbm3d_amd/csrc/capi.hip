@@ -358,9 +358,10 @@ void member_reach_steps(const vbm3d_params& p, unsigned maxflow, unsigned W, uns
 // The aggregation's grid boxes (k_aggregate's layout of AArgs::boxtab):
 // per crop, for each tile row and frame distance k the first grid row at
 // or below y0 - R_k - (K - 1) and the first beyond y0 + TY - 1 + R_k, then
-// the same for the tile columns; R_k = min(R, R0 + k Rs).
+// the same for the tile columns with the division by their count;
+// R_k = min(R, R0 + k Rs).
 void box_table(const GridDef& g, unsigned W, unsigned H, int K, unsigned Nf, int R, int R0, int Rs,
-               std::vector<int2>& box) {
+               std::vector<int4>& box) {
     const int nty = (int)((H + kAggTileY - 1) / kAggTileY), ntx = (int)((W + kAggTileX - 1) / kAggTileX);
     auto lb = [](const std::vector<unsigned>& v, long x) {
         size_t lo = 0, hi = v.size();
@@ -377,12 +378,14 @@ void box_table(const GridDef& g, unsigned W, unsigned H, int K, unsigned Nf, int
         for (int ty = 0; ty < nty; ty++)
             for (unsigned k = 0; k <= Nf; k++) {
                 const long Rk = std::min<long>(R, (long)R0 + (long)k * Rs), y0 = (long)ty * kAggTileY;
-                box.push_back(make_int2(lb(g.rows[c], y0 - Rk - (K - 1)), lb(g.rows[c], y0 + kAggTileY + Rk)));
+                box.push_back(make_int4(lb(g.rows[c], y0 - Rk - (K - 1)), lb(g.rows[c], y0 + kAggTileY + Rk), 0, 0));
             }
         for (int tx = 0; tx < ntx; tx++)
             for (unsigned k = 0; k <= Nf; k++) {
                 const long Rk = std::min<long>(R, (long)R0 + (long)k * Rs), x0 = (long)tx * kAggTileX;
-                box.push_back(make_int2(lb(g.cols[c], x0 - Rk - (K - 1)), lb(g.cols[c], x0 + kAggTileX + Rk)));
+                const int ja = lb(g.cols[c], x0 - Rk - (K - 1)), jb = lb(g.cols[c], x0 + kAggTileX + Rk);
+                const Div32 d = make_div32(jb > ja ? (unsigned)(jb - ja) : 1u);
+                box.push_back(make_int4(ja, jb, (int)d.m, (int)(d.s1 | (d.s2 << 8))));
             }
     }
 }
@@ -612,11 +615,11 @@ int run_step(vbm3d_ctx* ctx, const StepIO& io, hipStream_t s) {
     if (!gen) {
         // each tile's grid box per frame distance, as the kernel would
         // search it (one load per bound instead of a chain of dependent ones)
-        std::vector<int2> box;
+        std::vector<int4> box;
         box_table(g, io.W, io.H, (int)K, p.Nf, ag.R, ag.R0, ag.Rs, box);
-        HIP_TRY(ensure(ctx->boxtab, box.size() * sizeof(int2)));
-        HIP_TRY(hipMemcpyAsync(ctx->boxtab.p, box.data(), box.size() * sizeof(int2), hipMemcpyHostToDevice, s));
-        ag.boxtab = (const int2*)ctx->boxtab.p;
+        HIP_TRY(ensure(ctx->boxtab, box.size() * sizeof(int4)));
+        HIP_TRY(hipMemcpyAsync(ctx->boxtab.p, box.data(), box.size() * sizeof(int4), hipMemcpyHostToDevice, s));
+        ag.boxtab = (const int4*)ctx->boxtab.p;
         ag.nty = (int)((io.H + kAggTileY - 1) / kAggTileY);
         ag.ntx = (int)((io.W + kAggTileX - 1) / kAggTileX);
     }
@@ -1316,7 +1319,7 @@ size_t step_scratch_bytes(const vbm3d_params& p, unsigned W, unsigned H, unsigne
                             : g.rows.size() * (size_t)((H + kAggTileY - 1) / kAggTileY + (W + kAggTileX - 1) / kAggTileX) *
                                   (p.Nf + 1);
     return al(sizeof(Tabs)) + (gen ? al(sizeof(TabsG)) : 0) + al(nrf * 4) + al(g.rows.size() * 5 * 4) + al(npos * 2) +
-           al(nbox * sizeof(int2)) + slots * ((size_t)C * p.N * p.k * p.k * p.kt + p.N + 1 + C + mpos_words) * 4 + 4096;
+           al(nbox * sizeof(int4)) + slots * ((size_t)C * p.N * p.k * p.k * p.kt + p.N + 1 + C + mpos_words) * 4 + 4096;
 }
 
 }  // namespace

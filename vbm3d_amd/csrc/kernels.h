@@ -166,10 +166,11 @@ struct AArgs {
     const Tabs* tabs;
     int R;                      // largest member-to-reference displacement
     int R0, Rs;                 // a member k frames away lies within min(R, R0 + k Rs)
-    // optional, per crop: the grid rows (ia, ib) of [tile row][k], then the
-    // grid columns (ja, jb) of [tile column][k], k = frame distance 0..Nf;
+    // optional, per crop: the grid rows (ia, ib, -, -) of [tile row][k],
+    // then the grid columns (ja, jb, m, s1 | s2 << 8) of [tile column][k],
+    // k = frame distance 0..Nf, (m, s1, s2) the division by jb - ja (or 1);
     // null: the kernel searches the grid itself
-    const int2* boxtab;
+    const int4* boxtab;
     int nty, ntx;               // tile rows and columns of a frame
     Div32 divW;                 // division by W
     unsigned t_begin;           // first output frame of this launch

@@ -81,7 +81,7 @@ __device__ __forceinline__ void agg_tile(const A& a, const int bx, const int by,
     // column} of the grid patches whose members can reach the tile, and the
     // division by the column count
     __shared__ int4 s_box[2 * kGenMaxNf + 1];
-    __shared__ int s_bnd[4 * (2 * kGenMaxNf + 1)];
+    __shared__ int s_bnd[6 * (2 * kGenMaxNf + 1)];
     __shared__ Div32 s_div[2 * kGenMaxNf + 1];
     __shared__ int s_total;
 
@@ -97,13 +97,19 @@ __device__ __forceinline__ void agg_tile(const A& a, const int bx, const int by,
         // frame, row, column).
         __syncthreads();  // the previous crop's table is no longer read
         if (a.boxtab) {
-            // the host's table of the same bounds: one load per bound
-            if (tid < 4 * nfr) {
-                const int q = tid >> 2, which = tid & 3;
+            // the host's table of the same bounds: one load per bound, the
+            // columns' division with them
+            if (tid < 2 * nfr) {
+                const int q = tid >> 1, col = tid & 1;
                 const int k = abs(trlo + q - (int)to);
-                const size_t tl = (size_t)ci * (a.nty + a.ntx) + (which < 2 ? by : a.nty + bx);
-                const int2 v = a.boxtab[tl * (a.Nf + 1) + k];
-                s_bnd[tid] = (which & 1) ? v.y : v.x;
+                const size_t tl = (size_t)ci * (a.nty + a.ntx) + (col ? a.nty + bx : by);
+                const int4 v = a.boxtab[tl * (a.Nf + 1) + k];
+                s_bnd[6 * q + 2 * col] = v.x;
+                s_bnd[6 * q + 2 * col + 1] = v.y;
+                if (col) {
+                    s_bnd[6 * q + 4] = v.z;
+                    s_bnd[6 * q + 5] = v.w;
+                }
             }
         } else if (tid < 4 * nfr) {
             // the four searches of a frame run in four threads (each is a
@@ -114,24 +120,34 @@ __device__ __forceinline__ void agg_tile(const A& a, const int bx, const int by,
             const bool row = which < 2;
             const int lo = row ? y0 - Rk - (K - 1) : x0 - Rk - (K - 1);
             const int hi = row ? y0 + TY + Rk : x0 + TX + Rk;
-            s_bnd[tid] = lower_bound_u16(row ? rows : cols, row ? nrows : ncols, (which & 1) ? hi : lo);
+            s_bnd[6 * q + which] = lower_bound_u16(row ? rows : cols, row ? nrows : ncols, (which & 1) ? hi : lo);
         }
         __syncthreads();
-        if (tid < nfr) {
-            const int ia = s_bnd[4 * tid], ib = s_bnd[4 * tid + 1], ja = s_bnd[4 * tid + 2], jb = s_bnd[4 * tid + 3];
-            const int nr = max(ib - ia, 0), nc = max(jb - ja, 0);
-            s_box[tid] = make_int4(nr * nc, nc, ia, ja);
-            s_div[tid] = make_div32(nc > 0 ? nc : 1);
-        }
-        __syncthreads();
-        if (tid == 0) {
-            int acc = 0;
-            for (int q = 0; q < nfr; q++) {
-                const int cnt = s_box[q].x;
-                s_box[q].x = acc;
-                acc += cnt;
+        // wavefront 0: lane q < nfr assembles frame q's box; the boxes'
+        // first indices are an exclusive scan of their patch counts
+        if (wv == 0) {
+            int cnt = 0, nc = 0, ia = 0, ja = 0;
+            if (lane < nfr) {
+                ia = s_bnd[6 * lane];
+                ja = s_bnd[6 * lane + 2];
+                const int nr = max(s_bnd[6 * lane + 1] - ia, 0);
+                nc = max(s_bnd[6 * lane + 3] - ja, 0);
+                cnt = nr * nc;
+                if (a.boxtab) {
+                    const unsigned sh = (unsigned)s_bnd[6 * lane + 5];
+                    s_div[lane] = Div32{(uint32_t)s_bnd[6 * lane + 4], sh & 0xFFu, sh >> 8};
+                } else {
+                    s_div[lane] = make_div32(nc > 0 ? nc : 1);
+                }
             }
-            s_total = acc;
+            int incl = cnt;
+#pragma unroll
+            for (int o = 1; o < 32; o <<= 1) {
+                const int y = __shfl_up(incl, o, 64);
+                if (lane >= o) incl += y;
+            }
+            if (lane < nfr) s_box[lane] = make_int4(incl - cnt, nc, ia, ja);
+            if (lane == nfr - 1) s_total = incl;
         }
         __syncthreads();
         const long total = s_total;
