@@ -163,20 +163,22 @@ __device__ __forceinline__ void filter_one(const A& a, const long r, float* X, c
     const int nl = lane / K, rr = lane - nl * K;    // batch patch, row / column index
     const bool lact = lane < PB * K;
     float wsel = 0.f;  // step 1: lane c ends with channel c's weight
-    // row rr of member nl of the first lane batch: its base offset, and the
-    // row of the next (channel, source) pair fetched ahead of its use
-    const bool act0 = lact && nl < nsx;
-    unsigned mb = 0;
-    if (act0) {
-        const unsigned mi = mem[nl];
+    // mb0: offset of row rr of member nl, the lane's row in the first lane
+    // batch (its index is read whether or not the member exists, so that
+    // the load does not wait for the group size; every slot holds N
+    // entries).  The first batch's row of each (channel, source) pair is
+    // fetched one pair ahead of its use.
+    auto decode = [&](unsigned mi) {
         const unsigned pt = div32(mi, dWHC);
         const unsigned py = div32(mi - pt * whc, dW), px = mi - pt * whc - py * W;
-        mb = pt * whc + (py + rr) * W + px;
-    }
+        return pt * whc + (py + rr) * W + px;
+    };
+    const unsigned mb0 = decode(mem[min(nl, a.N - 1)]);
+    const bool act0 = lact && nl < nsx;
     float xn[K];
     auto fetch = [&](int c, int g) {
         if (act0) {
-            const float* row = (g == 0 ? a.noisy : a.basic) + mb + c * wh;
+            const float* row = (g == 0 ? a.noisy : a.basic) + mb0 + c * wh;
 #pragma unroll
             for (int q = 0; q < K; q++) xn[q] = row[q];
         }
@@ -202,10 +204,7 @@ __device__ __forceinline__ void filter_one(const A& a, const long r, float* X, c
                         if (STEP == 2 && g == 0) fetch(c, 1);
                         else if (c + 1 < C) fetch(c + 1, 0);
                     } else {
-                        const unsigned mi = mem[n];
-                        const unsigned pt = div32(mi, dWHC);
-                        const unsigned py = div32(mi - pt * whc, dW), px = mi - pt * whc - py * W;
-                        const float* row = src + pt * whc + c * wh + (py + rr) * W + px;
+                        const float* row = src + decode(mem[n]) + c * wh;
 #pragma unroll
                         for (int q = 0; q < K; q++) x[q] = row[q];
                     }
