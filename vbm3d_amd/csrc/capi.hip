@@ -578,6 +578,7 @@ int run_step(vbm3d_ctx* ctx, const StepIO& io, hipStream_t s) {
     bm.dist_out = nullptr;
     bm.nsim_out = nullptr;
     bm.nsx_out = (uint32_t*)ctx->nsx.p;
+    bm.band = 1;
     bm.count_out = nullptr;
     if (ctx->count) {
         HIP_TRY(ensure(ctx->counts, (size_t)S * nrf * 2 * 4));

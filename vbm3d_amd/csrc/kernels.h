@@ -131,6 +131,7 @@ struct BMArgs {
     Div32 dW, dWHC;       // division by W and by W*H*C
     uint32_t* count_out;  // null, or per patch of the launch, at 2*(r - r_begin):
                           // candidates evaluated, group size
+    int band;             // grid mode: workgroups in band order (band_order)
 };
 
 struct FArgs {

@@ -567,7 +567,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(bm_waves_fl
     const int lane = threadIdx.x & 63;
     const int w = threadIdx.x >> 6;
     const int nw = blockDim.x >> 6;
-    const long r = a.r_begin + (long)blockIdx.x * nw + w;
+    unsigned lb = blockIdx.x;
+    if (a.band) lb = band_order(lb, gridDim.x, max(1u, a.nrf / (unsigned)(8 * nw)));
+    const long r = a.r_begin + (long)lb * nw + w;
     if (r >= a.r_end) return;  // wave-uniform
     bm_one<K, C, SH>(a, r, smem + (size_t)w * L.total, lane);
 }
