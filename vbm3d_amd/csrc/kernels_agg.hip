@@ -270,7 +270,11 @@ __device__ __forceinline__ void agg_tile(const A& a, const int bx, const int by,
                         for (int c = 0; c < C; c++) w[u][c] = RAW ? 0.f : wb[c];
                         const int dx = px - (int)(pos & 0xFFFFu), dy = py - (int)(pos >> 16);
                         on[u] = (jj[u] >= 0) & inside & ((unsigned)dx < (unsigned)K) & ((unsigned)dy < (unsigned)K);
-                        const unsigned kk = (unsigned)(min(max(dy, 0), K - 1) * K + min(max(dx, 0), K - 1));
+                        // clamp to [0, K-1] in one instruction per axis
+                        int cx, cy;
+                        asm("v_med3_i32 %0, %1, 0, %2" : "=v"(cx) : "v"(dx), "s"(K - 1));
+                        asm("v_med3_i32 %0, %1, 0, %2" : "=v"(cy) : "v"(dy), "s"(K - 1));
+                        const unsigned kk = __umul24((unsigned)cy, (unsigned)K) + (unsigned)cx;
                         kai[u] = s_kai[kk];
                         // wave-uniform base, 32-bit lane byte offset; the
                         // plane index slot * C * N + n fits 32 bits (the
