@@ -301,9 +301,9 @@ __device__ __forceinline__ void bm_one(const A& a, const long r, float* base, co
                             }
                         }
                     }
-                __builtin_amdgcn_s_waitcnt(0);
+                // no wait here: the copies land while the slot list below,
+                // which does not read the tile, is formed
             }
-            wave_sync();
             // slot lane of chain h; a valid slot keeps its tile offset (bit
             // 31: the reference's own position) in s_d until its distance
             // replaces it
@@ -333,6 +333,7 @@ __device__ __forceinline__ void bm_one(const A& a, const long r, float* base, co
                 bal[h] = __ballot(valid);
                 if (valid) lst[(h ? __popcll(bal[0]) : 0) + lanes_below(bal[h], lane)] = (unsigned)(h * 64 + lane);
             }
+            __builtin_amdgcn_s_waitcnt(0);  // the window copies
             wave_sync();
             const int nv = __popcll(bal[0]) + __popcll(bal[1]);
             for (int e0 = 0; e0 < nv; e0 += 64) {
