@@ -212,7 +212,10 @@ __device__ __forceinline__ void bm_one(const A& a, const long r, float* base, co
         uint32_t* lst = (uint32_t*)c_d;  // c_d and c_i, adjacent: 128 entries
         // windows of frame f of both chains: flow projection of the chain's
         // current matches (x first, then y at the updated x), clamped
-        // Npr x Npr box in the next frame; lanes < 2 Nb
+        // Npr x Npr box in the next frame.  Lane wn < 2 Nb keeps window wn
+        // in registers (x0, y0, width, height, frame), which the copies and
+        // the slot list read with readlane: no LDS round trip.
+        int wx0 = 0, wy0 = 0, wwx = 0, wwy = 0, wfr = 0;
         auto windows = [&](int f) {
             if (lane < 2 * Nb) {
                 const int ch = lane >= Nb ? 1 : 0, m = lane - ch * Nb;
@@ -229,11 +232,11 @@ __device__ __forceinline__ void bm_one(const A& a, const long r, float* base, co
                     const float fy = fl ? fl[(size_t)ft * 2 * wh + wh + (py + hk) * W + px + hk] : 0.f;
                     py = (unsigned)min(max((int)((float)py + roundf(fy)), 0), (int)(H - K));
                     const int x0 = max(0, (int)px - hs), y0 = max(0, (int)py - hs);
-                    win[lane * 5 + 0] = x0;
-                    win[lane * 5 + 1] = y0;
-                    win[lane * 5 + 2] = min((int)W - K, (int)px + hs) - x0 + 1;
-                    win[lane * 5 + 3] = min((int)H - K, (int)py + hs) - y0 + 1;
-                    win[lane * 5 + 4] = (int)(pt + dir);
+                    wx0 = x0;
+                    wy0 = y0;
+                    wwx = min((int)W - K, (int)px + hs) - x0 + 1;
+                    wwy = min((int)H - K, (int)py + hs) - y0 + 1;
+                    wfr = (int)(pt + dir);
                 }
             }
             wave_sync();
@@ -271,11 +274,11 @@ __device__ __forceinline__ void bm_one(const A& a, const long r, float* base, co
                     for (int m = 0; m < 2; m++) {
                         if (!act[ch] || m >= Nb) continue;
                         const int wn = ch * Nb + m;
-                        const int aw = __builtin_amdgcn_readfirstlane(win[wn * 5 + 2]) + K - 1;
-                        const int ah = __builtin_amdgcn_readfirstlane(win[wn * 5 + 3]) + K - 1;
-                        const unsigned fb = (unsigned)__builtin_amdgcn_readfirstlane(win[wn * 5 + 4]) * whc +
-                                            (unsigned)__builtin_amdgcn_readfirstlane(win[wn * 5 + 1]) * W +
-                                            (unsigned)__builtin_amdgcn_readfirstlane(win[wn * 5]);
+                        const int aw = __builtin_amdgcn_readlane(wwx, wn) + K - 1;
+                        const int ah = __builtin_amdgcn_readlane(wwy, wn) + K - 1;
+                        const unsigned fb = (unsigned)__builtin_amdgcn_readlane(wfr, wn) * whc +
+                                            (unsigned)__builtin_amdgcn_readlane(wy0, wn) * W +
+                                            (unsigned)__builtin_amdgcn_readlane(wx0, wn);
                         // one scalar base per channel, kept opaque so that
                         // the copies take the scalar-base form (no 64-bit
                         // vector address arithmetic per copy)
@@ -311,12 +314,13 @@ __device__ __forceinline__ void bm_one(const A& a, const long r, float* base, co
 #pragma unroll
             for (int h = 0; h < 2; h++) {
                 // the chain's windows 0 and 1 (window 0 again when Nb = 1),
-                // read at wave-uniform addresses; lanes of window 1 select
-                // its values, and a candidate inside window 0 was seen
-                const int* wa = win + (h * Nb) * 5;
-                const int* wb = win + (h * Nb + (Nb > 1 ? 1 : 0)) * 5;
-                const int ax0 = wa[0], ay0 = wa[1], awx = wa[2], awy = wa[3];
-                const int bx0 = wb[0], by0 = wb[1], bwx = wb[2], bwy = wb[3];
+                // wave-uniform; lanes of window 1 select its values, and a
+                // candidate inside window 0 was seen
+                const int ia = h * Nb, ib = h * Nb + (Nb > 1 ? 1 : 0);
+                const int ax0 = __builtin_amdgcn_readlane(wx0, ia), ay0 = __builtin_amdgcn_readlane(wy0, ia);
+                const int awx = __builtin_amdgcn_readlane(wwx, ia), awy = __builtin_amdgcn_readlane(wwy, ia);
+                const int bx0 = __builtin_amdgcn_readlane(wx0, ib), by0 = __builtin_amdgcn_readlane(wy0, ib);
+                const int bwx = __builtin_amdgcn_readlane(wwx, ib), bwy = __builtin_amdgcn_readlane(wwy, ib);
                 const bool m1 = sm == 1;
                 const int x0 = m1 ? bx0 : ax0, y0 = m1 ? by0 : ay0;
                 const int wx = m1 ? bwx : awx, wy = m1 ? bwy : awy;
@@ -328,7 +332,8 @@ __device__ __forceinline__ void bm_one(const A& a, const long r, float* base, co
                     const unsigned toff = (unsigned)(wn * L.ws + sly * L.tp1 + slx);
                     const bool self = (unsigned)qx == rx && (unsigned)qy == ry;
                     s_d[h * 64 + lane] = __uint_as_float(toff | (self ? 0x80000000u : 0u));
-                    s_i[h * 64 + lane] = (unsigned)(m1 ? wb[4] : wa[4]) * whc + qy * W + qx;
+                    const int fr = m1 ? __builtin_amdgcn_readlane(wfr, ib) : __builtin_amdgcn_readlane(wfr, ia);
+                    s_i[h * 64 + lane] = (unsigned)fr * whc + qy * W + qx;
                 }
                 bal[h] = __ballot(valid);
                 if (valid) lst[(h ? __popcll(bal[0]) : 0) + lanes_below(bal[h], lane)] = (unsigned)(h * 64 + lane);
