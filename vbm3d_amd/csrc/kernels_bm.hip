@@ -307,9 +307,10 @@ __device__ __forceinline__ void bm_one(const A& a, const long r, float* base, co
                 // no wait here: the copies land while the slot list below,
                 // which does not read the tile, is formed
             }
-            // slot lane of chain h; a valid slot keeps its tile offset (bit
-            // 31: the reference's own position) in s_d until its distance
-            // replaces it
+            // slot lane of chain h; a valid slot's list entry holds its slot
+            // (bits 16..22), the reference's own position (bit 15) and its
+            // tile offset (bits 0..14; at most 4 windows of 12 x 13 x 3
+            // words)
             unsigned long long bal[2];
 #pragma unroll
             for (int h = 0; h < 2; h++) {
@@ -327,26 +328,26 @@ __device__ __forceinline__ void bm_one(const A& a, const long r, float* base, co
                 const int qx = x0 + slx, qy = y0 + sly;
                 const bool seen = m1 & (qx >= ax0) & (qx < ax0 + awx) & (qy >= ay0) & (qy < ay0 + awy);
                 const bool valid = act[h] & (sm < Nb) & (sl < P2) & (slx < wx) & (sly < wy) & !seen;
+                unsigned ent = 0;
                 if (valid) {
                     const int wn = h * Nb + sm;
                     const unsigned toff = (unsigned)(wn * L.ws + sly * L.tp1 + slx);
                     const bool self = (unsigned)qx == rx && (unsigned)qy == ry;
-                    s_d[h * 64 + lane] = __uint_as_float(toff | (self ? 0x80000000u : 0u));
+                    ent = ((unsigned)(h * 64 + lane) << 16) | (self ? 0x8000u : 0u) | toff;
                     const int fr = m1 ? __builtin_amdgcn_readlane(wfr, ib) : __builtin_amdgcn_readlane(wfr, ia);
                     s_i[h * 64 + lane] = (unsigned)fr * whc + qy * W + qx;
                 }
                 bal[h] = __ballot(valid);
-                if (valid) lst[(h ? __popcll(bal[0]) : 0) + lanes_below(bal[h], lane)] = (unsigned)(h * 64 + lane);
+                if (valid) lst[(h ? __popcll(bal[0]) : 0) + lanes_below(bal[h], lane)] = ent;
             }
             __builtin_amdgcn_s_waitcnt(0);  // the window copies
             wave_sync();
             const int nv = __popcll(bal[0]) + __popcll(bal[1]);
             for (int e0 = 0; e0 < nv; e0 += 64) {
                 if (e0 + lane < nv) {
-                    const unsigned sl = lst[e0 + lane];
-                    const unsigned tb = __float_as_uint(s_d[sl]);
-                    const float sv = ssd(tile + (tb & 0x7FFFFFFFu), L.tp1, cs1);
-                    s_d[sl] = sv / norm - ((tb >> 31) ? a.d : 0.f);
+                    const unsigned e = lst[e0 + lane];
+                    const float sv = ssd(tile + (e & 0x7FFFu), L.tp1, cs1);
+                    s_d[e >> 16] = sv / norm - ((e & 0x8000u) ? a.d : 0.f);
                 }
             }
             wave_sync();
