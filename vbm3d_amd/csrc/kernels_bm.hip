@@ -541,8 +541,11 @@ __device__ __forceinline__ void bm_one(const A& a, const long r, float* base, co
     //    (src/vbm3d.cpp:1169-1184)
     const int nsim = min(a.N, nbest);
     wave_select(b_d, b_i, nbest, nsim, o_d, o_i, scr, lane);
-    int it = nsim - 1;
-    while (o_d[it] > a.tau && it > 0) it--;
+    // the reference walks down from nsim - 1 while the distance exceeds tau
+    // (stopping at 0): the highest pick that does not, found with one
+    // ballot instead of a chain of dependent reads
+    const unsigned long long keep = __ballot(lane < nsim && !(o_d[lane < nsim ? lane : 0] > a.tau));
+    const int it = keep ? 63 - __clzll((long long)keep) : 0;
     unsigned nsx = 1;
     while (nsx * 2 <= (unsigned)(it + 1)) nsx *= 2;
     if (lane < a.N) {
