@@ -56,7 +56,7 @@ struct FlowCtl {
     int active;   // 1 while the current warp iterates
     int iter;     // iterations done in the current warp
     float error;  // last mean squared change
-    int pad;
+    unsigned done;  // blocks of k_iter_dual_ctl finished in this iteration
 };
 
 __device__ __forceinline__ int clampi(int x, int n, bool& out) {
@@ -371,6 +371,44 @@ __global__ void k_ctl_start(FlowCtl* ctl) {
     ctl->active = 1;
     ctl->iter = 0;
     ctl->error = INFINITY;
+    ctl->done = 0;
+}
+
+// Parts 2 and 3 in one launch, for levels of at most kFuseBlocks blocks:
+// the dual update, then the workgroup that finishes last (one atomic per
+// workgroup on the control word) sums the partials and runs the stopping
+// test exactly as k_iter_ctl does (256 threads, the same order), so the
+// flow and the iteration count are unchanged.  No workgroup waits for
+// another.
+constexpr int kFuseBlocks = 1024;
+__global__ void k_iter_dual_ctl(FlowCtl* ctl, const float* __restrict__ u1, const float* __restrict__ u2,
+                                float* p11, float* p12, float* p21, float* p22, int nx, int ny, float taut,
+                                const double* __restrict__ part, int nparts, float size, float eps2) {
+    if (!ctl->active) return;  // uniform
+    const int j = blockIdx.x * BX + threadIdx.x, i = blockIdx.y * BY + threadIdx.y;
+    if (j < nx && i < ny) dual_px(u1, u2, p11, p12, p21, p22, i, j, nx, ny, taut);
+    const int t = threadIdx.y * BX + threadIdx.x;
+    __shared__ int s_last;
+    __syncthreads();
+    if (t == 0) s_last = atomicAdd(&ctl->done, 1u) == gridDim.x * gridDim.y - 1;
+    __syncthreads();
+    if (!s_last) return;
+    double acc = 0.0;
+    for (int q = t; q < nparts; q += BX * BY) acc += part[q];
+    for (int o = 32; o >= 1; o >>= 1) acc += __shfl_xor(acc, o, 64);
+    __shared__ double s[BX * BY / 64];
+    if ((t & 63) == 0) s[t >> 6] = acc;
+    __syncthreads();
+    if (t == 0) {
+        double tt = 0.0;
+        for (int q = 0; q < BX * BY / 64; q++) tt += s[q];
+        float err = (float)tt;
+        err /= size;
+        ctl->error = err;
+        ctl->iter += 1;
+        ctl->active = (err > eps2 && ctl->iter < kMaxIter) ? 1 : 0;
+        ctl->done = 0;
+    }
 }
 
 // Every iteration of one warp on a level of at most kSmallPx pixels (and
@@ -619,6 +657,11 @@ int tvl1_device(vbm3d_ctx* ctx, const float* I0, const float* I1, float* u, floa
                 for (int it = 0; it < batch; it++) {
                     hipLaunchKernelGGL(k_iter_primal, g, b, 0, s, ctl, L[k].u1, L[k].u2, wx, wy, grad, rho, p11, p12,
                                        p21, p22, part, lx, ly, l_t, theta);
+                    if (np <= kFuseBlocks) {
+                        hipLaunchKernelGGL(k_iter_dual_ctl, g, b, 0, s, ctl, L[k].u1, L[k].u2, p11, p12, p21, p22,
+                                           lx, ly, taut, part, np, (float)sz, eps2);
+                        continue;
+                    }
                     hipLaunchKernelGGL(k_iter_dual, g, b, 0, s, ctl, L[k].u1, L[k].u2, p11, p12, p21, p22, lx, ly,
                                        taut);
                     hipLaunchKernelGGL(k_iter_ctl, dim3(1), dim3(256), 0, s, ctl, part, np, (float)sz, eps2);
